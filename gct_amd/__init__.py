@@ -1,0 +1,404 @@
+"""Python host layer over libgct_amd.so, the MI355X FDK reconstruction library.
+
+The library's C ABI (include/gct_amd.h) is the product boundary; this module
+binds it with ctypes so tests, the benchmark and Python callers drive the same
+entry points a C or C++ caller would.  Device buffers are passed as raw
+pointers (torch tensors' data_ptr() on ROCm); host buffers as numpy arrays.
+
+Importing fails loudly when the library has not been built: there is no
+fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libgct_amd.so"
+
+# gct_geom_kind
+PARALLEL_1D, FAN_ARC_1D, FAN_LINE_1D, PARALLEL_2D, CONE_PARALLEL, CYLINDRICAL, FLAT = range(7)
+KIND_NAMES = ["parallel_1d", "fan_arc_1d", "fan_line_1d", "parallel_2d", "cone_parallel", "cylindrical", "flat"]
+# gct_window
+WINDOW_NONE, WINDOW_HANN, WINDOW_HAMMING, WINDOW_SHEPP_LOGAN = range(4)
+# gct_storage
+STORE_F32, STORE_F16, STORE_BF16 = range(3)
+# gct_setup_precision
+SETUP_AUTO, SETUP_F32, SETUP_F64 = range(3)
+# gct_mem_mode
+MEM_AUTO, MEM_HIGH, MEM_LOW = range(3)
+
+STATUS = {0: "ok", 1: "invalid argument", 2: "runtime error", 3: "out of memory", 4: "no device",
+          5: "unsupported", 6: "rejected", 7: "not ready"}
+
+
+def is_1d(kind: int) -> bool:
+    return kind <= FAN_LINE_1D
+
+
+class Sampler(C.Structure):
+    _fields_ = [("num_samples", C.c_uint32), ("sampling_width", C.c_float)]
+
+
+class ProjGeom(C.Structure):
+    _fields_ = [("rot_ang", C.c_float), ("src_det_dist", C.c_float), ("src_iso_dist", C.c_float),
+                ("det_row_center", C.c_float), ("det_col_center", C.c_float), ("src_pos", C.c_float * 3),
+                ("det_row_dir", C.c_float * 3), ("det_col_dir", C.c_float * 3), ("iso_ray_dir", C.c_float * 3)]
+
+
+PROJ_GEOM_FLOATS = C.sizeof(ProjGeom) // 4   # 17
+
+
+class CTGeom(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("det_row", Sampler), ("det_col", Sampler), ("num_proj", C.c_uint32),
+                ("proj_geom", C.POINTER(ProjGeom))]
+
+
+class ImageGeom(C.Structure):
+    _fields_ = [("dim", C.c_uint32), ("x", Sampler), ("y", Sampler), ("z", Sampler), ("mid_point", C.c_float * 3)]
+
+
+class RotationGeom(C.Structure):
+    _fields_ = [("dir", C.c_int32), ("rot_ang_incr", C.c_float), ("rot_ang_range", C.c_float),
+                ("num_proj_per_rot", C.c_uint32)]
+
+
+class ReconOpts(C.Structure):
+    _fields_ = [("cut_off_freq", C.c_float), ("window", C.c_int32), ("storage", C.c_int32),
+                ("setup_precision", C.c_int32), ("mem_mode", C.c_int32), ("device", C.c_int32),
+                ("z_unroll", C.c_uint32), ("view_chunk", C.c_uint32)]
+
+
+class CircularParams(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("dir", C.c_int32), ("init_ang", C.c_float), ("ang_incr", C.c_float),
+                ("num_proj", C.c_uint32), ("hel_pitch", C.c_float), ("det_row_center", C.c_float),
+                ("det_col_center", C.c_float), ("src_det_dist", C.c_float), ("src_iso_dist", C.c_float)]
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [("fft_len", C.c_uint32), ("filt_pitch", C.c_uint32), ("filt_rows_per_view", C.c_uint32),
+                ("z_unroll", C.c_uint32), ("filt_bytes", C.c_uint64), ("device_bytes", C.c_uint64),
+                ("updates", C.c_uint64), ("rot_ang_incr", C.c_float), ("rot_ang_range", C.c_float),
+                ("rot_dir", C.c_int32), ("setup_f64", C.c_int32), ("circular", C.c_int32), ("storage", C.c_int32)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("device", C.c_int32), ("hbm_budget_bytes", C.c_uint64), ("max_inflight", C.c_uint32),
+                ("num_streams", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("rejected", C.c_uint64), ("updates", C.c_uint64), ("peak_device_bytes", C.c_uint64),
+                ("peak_inflight", C.c_uint32), ("busy_seconds", C.c_double)]
+
+
+class GctError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m gct_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L: C.CDLL) -> None:
+    P = C.c_void_p
+    st = C.c_int
+    sig = {
+        "gct_abi_version": (C.c_uint32, []),
+        "gct_last_error": (C.c_char_p, []),
+        "gct_status_string": (C.c_char_p, [C.c_int]),
+        "gct_recon_opts_default": (None, [C.POINTER(ReconOpts)]),
+        "gct_device_count": (C.c_int, []),
+        "gct_geom_create_circular": (st, [C.POINTER(CircularParams), P]),
+        "gct_rotation_geom_compute": (st, [C.POINTER(CTGeom), C.POINTER(RotationGeom), P, P]),
+        "gct_check_axial_recon": (st, [C.POINTER(CTGeom), C.c_uint32, C.c_uint32, C.c_uint32]),
+        "gct_ramp_kernel": (st, [C.POINTER(CTGeom), C.c_float, C.c_int32, P]),
+        "gct_projection_matrix": (st, [C.POINTER(ProjGeom), C.c_float, C.c_float, P]),
+        "gct_projection_matrix_decompose": (st, [P, C.c_float, C.POINTER(ProjGeom), P]),
+        "gct_reconstruct": (st, [C.POINTER(CTGeom), P, C.POINTER(ImageGeom), P, C.POINTER(ReconOpts)]),
+        "gct_required_memory": (st, [C.POINTER(CTGeom), C.POINTER(ImageGeom), C.POINTER(ReconOpts),
+                                     C.POINTER(C.c_uint64)]),
+        "gct_plan_create": (st, [C.POINTER(CTGeom), C.POINTER(ImageGeom), C.POINTER(ReconOpts), C.POINTER(P)]),
+        "gct_plan_destroy": (None, [P]),
+        "gct_plan_get_info": (st, [P, C.POINTER(PlanInfo)]),
+        "gct_plan_filter": (st, [P, P, C.c_uint32, C.c_uint32, P]),
+        "gct_plan_backproject": (st, [P, P, C.c_uint32, C.c_uint32, C.c_int, C.c_int, P]),
+        "gct_plan_execute": (st, [P, P, P, P]),
+        "gct_plan_load_filtered": (st, [P, P, P]),
+        "gct_plan_store_filtered": (st, [P, P, P]),
+        "gct_plan_forward_project": (st, [P, P, P, P]),
+        "gct_plan_autotune": (st, [P, P, C.POINTER(C.c_uint32), C.POINTER(C.c_float)]),
+        "gct_batch_create": (st, [C.POINTER(P), C.c_uint32, C.POINTER(P)]),
+        "gct_batch_destroy": (None, [P]),
+        "gct_batch_execute": (st, [P, C.POINTER(P), C.POINTER(P), C.c_int, P]),
+        "gct_server_create": (st, [C.POINTER(ServerOpts), C.POINTER(P)]),
+        "gct_server_destroy": (None, [P]),
+        "gct_server_submit": (st, [P, C.POINTER(CTGeom), C.POINTER(ImageGeom), C.POINTER(ReconOpts), P, P, C.c_int,
+                                   C.POINTER(C.c_uint64)]),
+        "gct_server_wait": (st, [P, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "gct_server_poll": (st, [P, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "gct_server_get_stats": (st, [P, C.POINTER(ServerStats)]),
+        "gct_phantom_random": (st, [C.c_uint64, C.c_uint32, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                    C.c_double, P]),
+        "gct_phantom_project": (st, [C.POINTER(CTGeom), C.c_int, P, C.c_uint32, C.c_double, C.c_uint64, P, C.c_int]),
+        "gct_phantom_rasterize": (st, [C.POINTER(ImageGeom), P, C.c_uint32, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name, None)
+        if f is None:
+            continue
+        f.restype = res
+        f.argtypes = args
+
+
+def check(status: int) -> None:
+    if status != 0:
+        raise GctError(status, lib().gct_last_error().decode())
+
+
+def _ptr(a) -> int | None:
+    """Address of a numpy array, a torch tensor or an int."""
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return a
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()
+
+
+# ---- geometry --------------------------------------------------------------
+
+class Scan:
+    """An AxialCTGeom: detector samplers plus one ProjGeom per view.
+
+    views is a float32 array [V][17] with gct_proj_geom's layout."""
+
+    def __init__(self, kind: int, nu: int, du: float, nv: int = 1, dv: float = 1.0, views=None):
+        self.kind = int(kind)
+        self.nu, self.du = int(nu), float(du)
+        self.nv, self.dv = (1, 1.0) if is_1d(kind) else (int(nv), float(dv))
+        self.views = np.ascontiguousarray(views, dtype=np.float32).reshape(-1, PROJ_GEOM_FLOATS)
+        self._c = None
+
+    @property
+    def num_proj(self) -> int:
+        return self.views.shape[0]
+
+    def c(self) -> CTGeom:
+        g = CTGeom()
+        g.kind = self.kind
+        g.det_row = Sampler(self.nu, self.du)
+        g.det_col = Sampler(self.nv, self.dv)
+        g.num_proj = self.num_proj
+        g.proj_geom = C.cast(self.views.ctypes.data, C.POINTER(ProjGeom))
+        self._c = g
+        return g
+
+    @property
+    def proj_shape(self):
+        return (self.num_proj, self.nv, self.nu)
+
+
+def circular_scan(kind: int, nu: int, du: float, num_proj: int, ang_incr: float, init_ang: float = 0.0,
+                  nv: int = 1, dv: float = 1.0, det_row_center: float | None = None,
+                  det_col_center: float | None = None, src_det_dist: float = 0.0, src_iso_dist: float = 0.0,
+                  direction: int = 1) -> Scan:
+    """ProjGeomFactory<...>::create for every view of an ideal circular scan."""
+    p = CircularParams()
+    p.kind, p.dir = kind, direction
+    p.init_ang, p.ang_incr, p.num_proj, p.hel_pitch = init_ang, ang_incr, num_proj, 0.0
+    p.det_row_center = (nu - 1) / 2.0 if det_row_center is None else det_row_center
+    p.det_col_center = (nv - 1) / 2.0 if det_col_center is None else det_col_center
+    p.src_det_dist, p.src_iso_dist = src_det_dist, src_iso_dist
+    views = np.zeros((num_proj, PROJ_GEOM_FLOATS), np.float32)
+    check(lib().gct_geom_create_circular(C.byref(p), views.ctypes.data))
+    return Scan(kind, nu, du, nv, dv, views)
+
+
+class Image:
+    def __init__(self, nx: int, ny: int, dx: float, dy: float, nz: int = 0, dz: float = 1.0, mid=(0.0, 0.0, 0.0)):
+        self.dim = 3 if nz else 2
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz) if nz else 1
+        self.dx, self.dy, self.dz = float(dx), float(dy), float(dz)
+        self.mid = tuple(float(m) for m in mid) + (0.0,) * (3 - len(mid))
+
+    def c(self) -> ImageGeom:
+        g = ImageGeom()
+        g.dim = self.dim
+        g.x = Sampler(self.nx, self.dx)
+        g.y = Sampler(self.ny, self.dy)
+        g.z = Sampler(self.nz if self.dim == 3 else 1, self.dz)
+        g.mid_point[:] = self.mid
+        return g
+
+    @property
+    def shape(self):
+        return (self.nz, self.ny, self.nx) if self.dim == 3 else (self.ny, self.nx)
+
+    @property
+    def nvox(self) -> int:
+        return self.nx * self.ny * (self.nz if self.dim == 3 else 1)
+
+
+def opts(**kw) -> ReconOpts:
+    o = ReconOpts()
+    lib().gct_recon_opts_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def rotation_geom(scan: Scan):
+    r = RotationGeom()
+    rot = np.zeros(scan.num_proj, np.float32)
+    rel = np.zeros(scan.num_proj, np.float32)
+    check(lib().gct_rotation_geom_compute(C.byref(scan.c()), C.byref(r), rot.ctypes.data, rel.ctypes.data))
+    return r, rot, rel
+
+
+def ramp_kernel(scan: Scan, cutoff: float = 1.0, window: int = WINDOW_NONE) -> np.ndarray:
+    taps = np.zeros(scan.nu, np.float32)
+    check(lib().gct_ramp_kernel(C.byref(scan.c()), cutoff, window, taps.ctypes.data))
+    return taps
+
+
+def reconstruct(scan: Scan, proj: np.ndarray, img: Image, **kw) -> np.ndarray:
+    """One-shot host-to-host reconstruction (Recon::CUDA::reconstruct)."""
+    proj = np.ascontiguousarray(proj, dtype=np.float32)
+    if proj.size != scan.num_proj * scan.nv * scan.nu:
+        raise ValueError("projection buffer does not match the scan")
+    vol = np.zeros(img.shape, np.float32)
+    o = opts(**kw)
+    check(lib().gct_reconstruct(C.byref(scan.c()), proj.ctypes.data, C.byref(img.c()), vol.ctypes.data, C.byref(o)))
+    return vol
+
+
+class Plan:
+    """Device-resident FDK work for one scan shape (gct_plan)."""
+
+    def __init__(self, scan: Scan, img: Image, **kw):
+        self.scan, self.img = scan, img
+        self._o = opts(**kw)
+        h = C.c_void_p()
+        check(lib().gct_plan_create(C.byref(scan.c()), C.byref(img.c()), C.byref(self._o), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gct_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> PlanInfo:
+        i = PlanInfo()
+        check(lib().gct_plan_get_info(self.h, C.byref(i)))
+        return i
+
+    def filter(self, proj, first: int = 0, n: int | None = None, stream=None):
+        n = self.scan.num_proj - first if n is None else n
+        check(lib().gct_plan_filter(self.h, _ptr(proj), first, n, _stream(stream)))
+
+    def backproject(self, vol, first: int = 0, n: int | None = None, accumulate=False, finish=True, stream=None):
+        n = self.scan.num_proj - first if n is None else n
+        check(lib().gct_plan_backproject(self.h, _ptr(vol), first, n, int(accumulate), int(finish), _stream(stream)))
+
+    def execute(self, proj, vol, stream=None):
+        check(lib().gct_plan_execute(self.h, _ptr(proj), _ptr(vol), _stream(stream)))
+
+    def load_filtered(self, filt, stream=None):
+        check(lib().gct_plan_load_filtered(self.h, _ptr(filt), _stream(stream)))
+
+    def store_filtered(self, filt, stream=None):
+        check(lib().gct_plan_store_filtered(self.h, _ptr(filt), _stream(stream)))
+
+    def forward_project(self, vol, proj, stream=None):
+        check(lib().gct_plan_forward_project(self.h, _ptr(vol), _ptr(proj), _stream(stream)))
+
+    def autotune(self, vol):
+        zu, ms = C.c_uint32(), C.c_float()
+        check(lib().gct_plan_autotune(self.h, _ptr(vol), C.byref(zu), C.byref(ms)))
+        return zu.value, ms.value
+
+
+class Batch:
+    def __init__(self, plans):
+        self.plans = list(plans)
+        arr = (C.c_void_p * len(self.plans))(*[p.h.value for p in self.plans])
+        h = C.c_void_p()
+        check(lib().gct_batch_create(arr, len(self.plans), C.byref(h)))
+        self.h = h
+
+    def execute(self, projs, vols, use_graph=True, stream=None):
+        pa = (C.c_void_p * len(projs))(*[_ptr(p) for p in projs])
+        va = (C.c_void_p * len(vols))(*[_ptr(v) for v in vols])
+        check(lib().gct_batch_execute(self.h, pa, va, int(use_graph), _stream(stream)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gct_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _stream(s):
+    if s is None:
+        return None
+    if isinstance(s, int):
+        return s
+    return s.cuda_stream
+
+
+# ---- phantoms ----------------------------------------------------------------
+
+def phantom_random(seed: int, count: int, dim: int, radius: float, axis=(0.05, 0.35), dens=(-0.2, 1.0)) -> np.ndarray:
+    p = np.zeros((count, 10), np.float64)
+    check(lib().gct_phantom_random(seed, count, dim, radius, axis[0], axis[1], dens[0], dens[1], p.ctypes.data))
+    return p
+
+
+def phantom_project(scan: Scan, params: np.ndarray, i0: float = 0.0, noise_seed: int = 0, nthreads: int = 0):
+    params = np.ascontiguousarray(params, np.float64)
+    proj = np.zeros(scan.proj_shape, np.float32)
+    dim = 2 if is_1d(scan.kind) else 3
+    check(lib().gct_phantom_project(C.byref(scan.c()), dim, params.ctypes.data, params.shape[0], i0, noise_seed,
+                                    proj.ctypes.data, nthreads))
+    return proj
+
+
+def phantom_rasterize(img: Image, params: np.ndarray) -> np.ndarray:
+    params = np.ascontiguousarray(params, np.float64)
+    vol = np.zeros(img.shape, np.float32)
+    check(lib().gct_phantom_rasterize(C.byref(img.c()), params.ctypes.data, params.shape[0], vol.ctypes.data))
+    return vol
+
+
+def device_count() -> int:
+    return int(lib().gct_device_count())

@@ -1,0 +1,173 @@
+// Internal declarations shared by the host sources and the HIP kernels of
+// libgct_amd.  Nothing here crosses the C ABI.
+#pragma once
+
+#include "gct_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace gct {
+
+// ---- errors -------------------------------------------------------------
+
+struct Error : std::runtime_error {
+    gct_status code;
+    Error(gct_status c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string &msg);
+
+[[noreturn]] inline void fail(gct_status c, const std::string &m) { throw Error(c, m); }
+
+inline void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) {
+        gct_status c = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? GCT_ERR_OUT_OF_MEMORY
+                                                                                    : GCT_ERR_RUNTIME;
+        throw Error(c, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+// Runs f, converting exceptions into a status and the thread's last error.
+template <class F>
+gct_status guarded(F &&f)
+{
+    try {
+        f();
+        return GCT_OK;
+    } catch (const Error &e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_last_error("host allocation failed");
+        return GCT_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        return GCT_ERR_RUNTIME;
+    }
+}
+
+// ---- geometry helpers (geometry.cpp) --------------------------------------
+
+__host__ __device__ inline bool is_1d(int kind) { return kind <= GCT_FAN_LINE_1D; }
+__host__ __device__ inline bool is_parallel_family(int kind)
+{
+    return kind == GCT_PARALLEL_1D || kind == GCT_PARALLEL_2D || kind == GCT_CONE_PARALLEL;
+}
+__host__ __device__ inline bool is_arc_row(int kind) { return kind == GCT_FAN_ARC_1D || kind == GCT_CYLINDRICAL; }
+
+struct RotationInfo {
+    int dir = 1;
+    float incr = 0.0f;
+    float range = 0.0f;
+    uint32_t nppr = 0;
+    std::vector<float> rot_ang;
+    std::vector<float> rel_ang;
+};
+
+// getRotationGeom; throws GCT_ERR_RUNTIME for an invalid trajectory.
+RotationInfo rotation_info(const gct_ct_geom &g);
+void validate_geom(const gct_ct_geom &g);
+void validate_image(const gct_ct_geom &g, const gct_image_geom &img);
+std::vector<double> ramp_taps_f64(int kind, uint32_t n, float width, float cutoff, int window,
+                                  std::vector<float> *taps32);
+uint32_t next_pow2(uint32_t n);
+
+// ---- device-side records --------------------------------------------------
+
+// Per-view backprojection record: 16 floats, read with one scalar load.
+//   u[4], v[4], w[4]: affine forms in voxel indices (1, ix, iy, iz)
+//   k[4]: kind-specific constants (see plan.cpp: build_bp_records)
+struct alignas(64) BpRec {
+    float u[4];
+    float v[4];
+    float w[4];
+    float k[4];
+};
+
+// The same record in FP64 for the FP64 setup path (volumes >= 512^3).
+struct alignas(64) BpRec64 {
+    double u[4];
+    double v[4];
+    double w[4];
+    double k[4];
+};
+
+// Per-view filter record: cosine-weight constants.
+struct alignas(16) FiltRec {
+    float cu, cv, sdd, pad;
+};
+
+// ---- kernel launchers (kernels.hip) ---------------------------------------
+
+struct FilterLaunch {
+    int kind;
+    uint32_t nu, nv, L;
+    uint32_t first_view, num_views;
+    float du, dv;
+    const float *proj;            // [V][Nv][Nu] (view index relative to proj base)
+    uint32_t proj_view_offset;    // view index of proj's first view
+    const FiltRec *frec;          // per view (absolute)
+    const float *red;             // [V][Nu] (absolute)
+    const float2 *H;              // [L]
+    const float2 *tw;             // [L]
+    void *filt;                   // padded output buffer (absolute)
+    uint32_t pitch, rows_pv;
+    int storage;
+};
+
+struct BpLaunch {
+    int kind;
+    int storage;
+    bool circular;
+    bool setup_f64;
+    uint32_t nx, ny, nz;
+    uint32_t nu, nv;
+    uint32_t first_view, num_views;
+    uint32_t pitch, rows_pv;
+    const void *filt;
+    const BpRec *rec;
+    const BpRec64 *rec64;
+    float *vol;
+    float inv_dgamma;
+    float scale;                  // applied when finish
+    bool accumulate, finish;
+    uint32_t z_unroll;
+};
+
+struct FpLaunch {
+    int kind;
+    uint32_t nx, ny, nz;
+    float x0, y0, z0, dx, dy, dz;
+    uint32_t nu, nv, nviews;
+    float du, dv;
+    const gct_proj_geom *views;   // device copy
+    const float *vol;
+    float *proj;
+};
+
+void launch_redundancy(int kind, const float *fan_ang, const float *beta, uint32_t nu, uint32_t nviews,
+                       float range, int parallel_value_mode, float parallel_value, float *red, hipStream_t s);
+void launch_filter(const FilterLaunch &a, hipStream_t s);
+void launch_backproject(const BpLaunch &a, hipStream_t s);
+void launch_forward_project(const FpLaunch &a, hipStream_t s);
+void launch_pack_filtered(const float *src, void *filt, int storage, uint32_t nu, uint32_t nv,
+                          uint32_t nviews, uint32_t pitch, uint32_t rows_pv, hipStream_t s);
+void launch_unpack_filtered(const void *filt, float *dst, int storage, uint32_t nu, uint32_t nv,
+                            uint32_t nviews, uint32_t pitch, uint32_t rows_pv, hipStream_t s);
+
+// One work item of the persistent batched backprojection: a z-block of one scan.
+struct BatchScan {
+    BpLaunch bp;                  // per-scan parameters (views: all)
+    uint32_t tiles_x, tiles_y, tiles_z;
+    uint32_t tile_begin;          // prefix sum over scans
+};
+void launch_backproject_batch(const BatchScan *scans_dev, uint32_t num_scans, uint32_t total_tiles,
+                              uint32_t *counter, hipStream_t s);
+
+}  // namespace gct

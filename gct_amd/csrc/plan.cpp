@@ -1,0 +1,907 @@
+// Host orchestration of the FDK path: plans (per-view tables, filter
+// spectrum, device buffers), the fused filter and backprojection stages, the
+// one-shot reconstruct with its high/low-memory paths, the autotuner and
+// batches captured in HIP graphs.
+//
+// Reference: Recon::CUDA::reconFDK and Detail::reconFDKHighMem/LowMem
+// (src/cuda/recon/fdk_axial.cuh:53-205), CTGeomDev (src/cuda/geom/ct_geom_dev.cu),
+// RampFilteringFFT (src/cuda/recon/ramp_filtering_fft.cuh:79-111),
+// getParkerWeights (src/cuda/recon/parker_weighting.cuh:132-200).
+#include "common.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <numbers>
+#include <tuple>
+
+namespace gct {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+namespace {
+
+// ---- device memory --------------------------------------------------------
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) : bytes(n)
+    {
+        if (n) hip_check(hipMalloc(&p, n), "hipMalloc");
+    }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p; bytes = o.bytes;
+            o.p = nullptr; o.bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct DeviceGuard {
+    int prev = 0;
+    explicit DeviceGuard(int dev)
+    {
+        hip_check(hipGetDevice(&prev), "hipGetDevice");
+        if (dev != prev) hip_check(hipSetDevice(dev), "hipSetDevice");
+    }
+    ~DeviceGuard() { (void)hipSetDevice(prev); }
+};
+
+// ---- host FFT (FP64) for the filter spectrum ------------------------------
+
+void fft64(std::vector<std::complex<double>> &a)
+{
+    const size_t n = a.size();
+    for (size_t i = 1, j = 0; i < n; ++i) {
+        size_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    for (size_t len = 2; len <= n; len <<= 1) {
+        const double ang = -2.0 * std::numbers::pi / static_cast<double>(len);
+        for (size_t i = 0; i < n; i += len)
+            for (size_t j = 0; j < len / 2; ++j) {
+                const std::complex<double> w(std::cos(ang * static_cast<double>(j)), std::sin(ang * static_cast<double>(j)));
+                const std::complex<double> u = a[i + j], v = a[i + j + len / 2] * w;
+                a[i + j] = u + v;
+                a[i + j + len / 2] = u - v;
+            }
+    }
+}
+
+size_t storage_bytes(int storage) { return storage == GCT_STORE_F32 ? 4u : 2u; }
+
+struct Tuning {
+    uint32_t z_unroll;
+};
+std::mutex g_tune_mu;
+std::map<std::tuple<int, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, int, int, int>, Tuning> g_tune;
+
+}  // namespace
+
+}  // namespace gct
+
+using namespace gct;
+
+// ---- the plan ---------------------------------------------------------------
+
+struct gct_plan {
+    int device = 0;
+    gct_ct_geom geom{};
+    std::vector<gct_proj_geom> views;
+    gct_image_geom img{};
+    gct_recon_opts opts{};
+    RotationInfo rot;
+    uint32_t nu = 0, nv = 0, V = 0, L = 0;
+    uint32_t pitch = 0, rows_pv = 0;
+    uint32_t filt_views = 0;   // views the filtered buffer holds
+    bool circular = true;
+    bool setup_f64 = false;
+    float inv_dgamma = 0.0f;
+    float scale = 1.0f;
+    uint32_t z_unroll = 8;
+    DevBuf d_brec, d_brec64, d_frec, d_red, d_H, d_tw, d_filt, d_views;
+};
+
+namespace gct {
+namespace {
+
+void build_records(gct_plan &p, std::vector<BpRec> &rec, std::vector<BpRec64> &rec64, std::vector<FiltRec> &frec)
+{
+    const gct_ct_geom &g = p.geom;
+    const gct_image_geom &im = p.img;
+    const double x0 = double(im.mid_point[0]) - (double(im.x.num_samples) - 1.0) * double(im.x.sampling_width) / 2.0;
+    const double y0 = double(im.mid_point[1]) - (double(im.y.num_samples) - 1.0) * double(im.y.sampling_width) / 2.0;
+    const double z0 = (im.dim == 3) ? double(im.mid_point[2]) - (double(im.z.num_samples) - 1.0) * double(im.z.sampling_width) / 2.0 : 0.0;
+    const double dx = im.x.sampling_width, dy = im.y.sampling_width, dz = (im.dim == 3) ? im.z.sampling_width : 0.0;
+    const double du = g.det_row.sampling_width, dv = is_1d(g.kind) ? 1.0 : g.det_col.sampling_width;
+    rec.assign(p.V, BpRec{});
+    rec64.assign(p.V, BpRec64{});
+    frec.assign(p.V, FiltRec{});
+    p.circular = true;
+    // affine form (c0 + c1 ix + c2 iy + c3 iz) of (position - src) . e, scaled by f
+    auto affine = [&](const double s[3], const double e[3], double f, double out[4]) {
+        out[0] = ((x0 - s[0]) * e[0] + (y0 - s[1]) * e[1] + (z0 - s[2]) * e[2]) * f;
+        out[1] = dx * e[0] * f;
+        out[2] = dy * e[1] * f;
+        out[3] = dz * e[2] * f;
+    };
+    for (uint32_t v = 0; v < p.V; ++v) {
+        const gct_proj_geom &pg = p.views[v];
+        BpRec64 r{};
+        const double cu = pg.det_row_center, cv = pg.det_col_center, D = pg.src_det_dist;
+        double s[3] = {pg.src_pos[0], pg.src_pos[1], is_1d(g.kind) ? 0.0 : pg.src_pos[2]};
+        double eu[3] = {pg.det_row_dir[0], pg.det_row_dir[1], is_1d(g.kind) ? 0.0 : pg.det_row_dir[2]};
+        double ev[3] = {pg.det_col_dir[0], pg.det_col_dir[1], pg.det_col_dir[2]};
+        double ew[3] = {pg.iso_ray_dir[0], pg.iso_ray_dir[1], is_1d(g.kind) ? 0.0 : pg.iso_ray_dir[2]};
+        frec[v] = FiltRec{pg.det_row_center, pg.det_col_center, pg.src_det_dist, 0.0f};
+        switch (g.kind) {
+        case GCT_PARALLEL_1D:
+        case GCT_PARALLEL_2D:
+        case GCT_CONE_PARALLEL: {
+            // t = x cos(theta + pi/2) + y sin(theta + pi/2) (coord_trans_proj.cuh:50-59);
+            // col = t / du + cu; 2-D rows: -z (:149) or -z D / SID (:168), row = . / dv + cv
+            const double th = double(pg.rot_ang) + std::numbers::pi / 2.0;
+            const double c = std::cos(th), sn = std::sin(th);
+            r.u[0] = cu + (x0 * c + y0 * sn) / du;
+            r.u[1] = dx * c / du;
+            r.u[2] = dy * sn / du;
+            const double m = (g.kind == GCT_CONE_PARALLEL) ? D / double(pg.src_iso_dist) : 1.0;
+            r.v[0] = cv - z0 * m / dv;
+            r.v[3] = -dz * m / dv;
+            break;
+        }
+        case GCT_FAN_LINE_1D:
+        case GCT_FLAT: {
+            // u = D (r.eu)/(r.ew), v = D (r.ev)/(r.ew), w = D SID/(r.ew)^2
+            // (coord_trans_proj.cuh:108-134, 211-238): every form is divided by
+            // sqrt(D SID) so that 1/(W.X)^2 is the weight directly.
+            if (g.kind == GCT_FAN_LINE_1D) { ew[0] = -eu[1]; ew[1] = eu[0]; ew[2] = 0.0; }   // rot90 (ct_geom_dev.cu:154-156)
+            else {
+                ew[0] = eu[1] * ev[2] - eu[2] * ev[1];   // eu x ev (ct_geom_dev.cu:407-411)
+                ew[1] = eu[2] * ev[0] - eu[0] * ev[2];
+                ew[2] = eu[0] * ev[1] - eu[1] * ev[0];
+            }
+            const double sid = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+            const double k = 1.0 / std::sqrt(D * sid);
+            affine(s, ew, k, r.w);
+            affine(s, eu, D / du * k, r.u);
+            if (g.kind == GCT_FLAT) affine(s, ev, D / dv * k, r.v);
+            r.k[0] = cu;
+            r.k[1] = cv;
+            break;
+        }
+        case GCT_FAN_ARC_1D:
+        case GCT_CYLINDRICAL: {
+            // gamma = atan2(r.eu, r.ew); v = D (r^.ev)/sqrt(1 - (r^.ev)^2) = D (r.ev)/rho
+            // for an orthonormal basis; w = SID / |r|^2 (coord_trans_proj.cuh:80-105, 175-208)
+            const double sid = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+            affine(s, ew, 1.0, r.w);
+            affine(s, eu, 1.0, r.u);
+            if (g.kind == GCT_CYLINDRICAL) affine(s, ev, 1.0, r.v);
+            r.k[0] = cu;
+            r.k[1] = cv;
+            r.k[2] = D / dv;
+            r.k[3] = sid;
+            break;
+        }
+        }
+        rec64[v] = r;
+        BpRec &f = rec[v];
+        for (int i = 0; i < 4; ++i) {
+            f.u[i] = static_cast<float>(r.u[i]);
+            f.v[i] = static_cast<float>(r.v[i]);
+            f.w[i] = static_cast<float>(r.w[i]);
+            f.k[i] = static_cast<float>(r.k[i]);
+        }
+        if (g.kind == GCT_FLAT || g.kind == GCT_CYLINDRICAL)
+            if (f.u[3] != 0.0f || f.w[3] != 0.0f) p.circular = false;
+    }
+}
+
+// The arc/cylindrical mapping uses rho = |(r.eu, r.ew)|, which equals the
+// reference's |r| sqrt(1 - (r^.ev)^2) only for an orthonormal detector basis.
+void check_basis(const gct_plan &p)
+{
+    if (!is_arc_row(p.geom.kind)) return;
+    for (uint32_t v = 0; v < p.V; ++v) {
+        const gct_proj_geom &pg = p.views[v];
+        const int n = (p.geom.kind == GCT_CYLINDRICAL) ? 3 : 2;
+        auto dot = [&](const float *a, const float *b) {
+            double s = 0.0;
+            for (int i = 0; i < n; ++i) s += double(a[i]) * double(b[i]);
+            return s;
+        };
+        bool ok = std::fabs(dot(pg.det_row_dir, pg.det_row_dir) - 1.0) < 1e-5 &&
+                  std::fabs(dot(pg.iso_ray_dir, pg.iso_ray_dir) - 1.0) < 1e-5 &&
+                  std::fabs(dot(pg.det_row_dir, pg.iso_ray_dir)) < 1e-5;
+        if (n == 3)
+            ok = ok && std::fabs(dot(pg.det_col_dir, pg.det_col_dir) - 1.0) < 1e-5 &&
+                 std::fabs(dot(pg.det_col_dir, pg.det_row_dir)) < 1e-5 &&
+                 std::fabs(dot(pg.det_col_dir, pg.iso_ray_dir)) < 1e-5;
+        if (!ok) fail(GCT_ERR_UNSUPPORTED, "arc and cylindrical detectors need an orthonormal detector basis");
+    }
+}
+
+void upload(DevBuf &b, const void *src, size_t bytes)
+{
+    b = DevBuf(bytes);
+    hip_check(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+
+std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geom &img, const gct_recon_opts &o,
+                                    uint32_t filt_views)
+{
+    validate_geom(geom);
+    validate_image(geom, img);
+    if (geom.det_row.num_samples > 2048u) fail(GCT_ERR_UNSUPPORTED, "more than 2048 detector columns");
+    if (o.storage < GCT_STORE_F32 || o.storage > GCT_STORE_BF16) fail(GCT_ERR_INVALID_ARGUMENT, "unknown storage");
+    if (o.z_unroll != 0 && o.z_unroll != 4 && o.z_unroll != 8 && o.z_unroll != 16)
+        fail(GCT_ERR_INVALID_ARGUMENT, "z_unroll must be 0, 4, 8 or 16");
+    auto p = std::make_unique<gct_plan>();
+    p->device = o.device;
+    p->opts = o;
+    p->views.assign(geom.proj_geom, geom.proj_geom + geom.num_proj);
+    p->geom = geom;
+    p->geom.proj_geom = p->views.data();
+    p->img = img;
+    p->V = geom.num_proj;
+    p->nu = geom.det_row.num_samples;
+    p->nv = is_1d(geom.kind) ? 1u : geom.det_col.num_samples;
+    p->L = next_pow2(2u * p->nu);
+    p->rot = rotation_info(p->geom);   // throws for an invalid trajectory (ct_geom.h:26-32)
+    p->scale = (p->rot.incr > 0.0f) ? p->rot.incr : 1.0f;   // fdk_axial.cuh:196-199
+    p->inv_dgamma = static_cast<float>(1.0 / double(geom.det_row.sampling_width));
+    const uint64_t nvox = uint64_t(img.x.num_samples) * img.y.num_samples * (img.dim == 3 ? img.z.num_samples : 1u);
+    const bool cone = geom.kind == GCT_FLAT || geom.kind == GCT_CYLINDRICAL;
+    p->setup_f64 = cone && (o.setup_precision == GCT_SETUP_F64 ||
+                            (o.setup_precision == GCT_SETUP_AUTO && nvox >= uint64_t(512) * 512 * 512));
+    p->pitch = (p->nu + 3u + 15u) & ~15u;
+    p->rows_pv = p->nv + 3u;
+    p->filt_views = filt_views ? std::min(filt_views, p->V) : p->V;
+    check_basis(*p);
+
+    DeviceGuard dg(p->device);
+    std::vector<BpRec> rec;
+    std::vector<BpRec64> rec64;
+    std::vector<FiltRec> frec;
+    build_records(*p, rec, rec64, frec);
+    upload(p->d_brec, rec.data(), rec.size() * sizeof(BpRec));
+    if (p->setup_f64) upload(p->d_brec64, rec64.data(), rec64.size() * sizeof(BpRec64));
+    upload(p->d_frec, frec.data(), frec.size() * sizeof(FiltRec));
+    upload(p->d_views, p->views.data(), p->views.size() * sizeof(gct_proj_geom));
+
+    // Filter spectrum: FFT_L of the N-tap kernel placed at n mod L, scaled by
+    // 1/L (the cuFFT C2R normalisation of ramp_filtering_fft.cu:59), from the
+    // FP32 taps the reference uses, evaluated in FP64 and stored FP32.
+    std::vector<float> taps32;
+    ramp_taps_f64(geom.kind, p->nu, geom.det_row.sampling_width, o.cut_off_freq, o.window, &taps32);
+    std::vector<std::complex<double>> h(p->L, 0.0);
+    const int half = static_cast<int>(p->nu) / 2;
+    for (uint32_t k = 0; k < p->nu; ++k) {
+        const int m = static_cast<int>(k) - half;
+        h[static_cast<uint32_t>((m + static_cast<int>(p->L)) % static_cast<int>(p->L))] = taps32[k];
+    }
+    fft64(h);
+    std::vector<float2> H(p->L), tw(p->L);
+    for (uint32_t k = 0; k < p->L; ++k) {
+        H[k] = make_float2(static_cast<float>(h[k].real() / p->L), static_cast<float>(h[k].imag() / p->L));
+        const double a = -2.0 * std::numbers::pi * static_cast<double>(k) / static_cast<double>(p->L);
+        tw[k] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+    }
+    upload(p->d_H, H.data(), H.size() * sizeof(float2));
+    upload(p->d_tw, tw.data(), tw.size() * sizeof(float2));
+
+    // Redundancy table [V][Nu] (getParkerWeights, parker_weighting.cuh:132-200):
+    // fan angles from view 0 on the host in FP32 as the reference does
+    // (fanAngles :61-77), the table on the device.
+    p->d_red = DevBuf(size_t(p->V) * p->nu * sizeof(float));
+    {
+        int mode = 0;
+        float value = 1.0f;
+        std::vector<float> fan(p->nu, 0.0f), beta(p->V, 0.0f);
+        if (is_parallel_family(geom.kind)) {
+            const float pi_f = std::numbers::pi_v<float>;
+            value = (p->rot.incr > 0.0f && p->rot.range <= (p->rot.incr + pi_f)) ? 1.0f : 0.5f;
+        } else if (p->rot.range > std::numbers::pi_v<float>) {
+            mode = 1;
+            const gct_proj_geom &v0 = p->views[0];
+            for (uint32_t c = 0; c < p->nu; ++c) {
+                const float d = (static_cast<float>(c) - v0.det_row_center) * geom.det_row.sampling_width;
+                fan[c] = is_arc_row(geom.kind) ? d : std::atan2(d, v0.src_det_dist);
+            }
+            for (uint32_t v = 0; v < p->V; ++v) beta[v] = (p->rot.dir < 0) ? p->rot.rel_ang[p->V - 1 - v] : p->rot.rel_ang[v];
+        }
+        // range <= pi for a fan geometry: the reference leaves the table
+        // uninitialised (parker_weighting.cuh:163-175); this library uses 1.
+        DevBuf dfan, dbeta;
+        upload(dfan, fan.data(), fan.size() * sizeof(float));
+        upload(dbeta, beta.data(), beta.size() * sizeof(float));
+        launch_redundancy(geom.kind, dfan.as<float>(), dbeta.as<float>(), p->nu, p->V, p->rot.range, mode, value,
+                          p->d_red.as<float>(), nullptr);
+        hip_check(hipDeviceSynchronize(), "redundancy weights");
+    }
+
+    const size_t fbytes = size_t(p->filt_views) * p->rows_pv * p->pitch * storage_bytes(o.storage);
+    p->d_filt = DevBuf(fbytes);
+    hip_check(hipMemset(p->d_filt.p, 0, fbytes), "hipMemset");   // zero padding stays zero
+
+    p->z_unroll = o.z_unroll ? o.z_unroll : 8u;
+    if (!o.z_unroll) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find({geom.kind, img.x.num_samples, img.y.num_samples, img.dim == 3 ? img.z.num_samples : 1u,
+                               p->V, p->nu, p->nv, o.storage, int(p->circular), int(p->setup_f64)});
+        if (it != g_tune.end()) p->z_unroll = it->second.z_unroll;
+    }
+    return p;
+}
+
+FilterLaunch filter_args(gct_plan &p, const float *proj, uint32_t proj_view_offset, uint32_t first, uint32_t n,
+                         uint32_t filt_view_offset)
+{
+    FilterLaunch a{};
+    a.kind = p.geom.kind;
+    a.nu = p.nu;
+    a.nv = p.nv;
+    a.L = p.L;
+    a.first_view = first;
+    a.num_views = n;
+    a.du = p.geom.det_row.sampling_width;
+    a.dv = is_1d(p.geom.kind) ? 1.0f : p.geom.det_col.sampling_width;
+    a.proj = proj;
+    a.proj_view_offset = proj_view_offset;
+    a.frec = p.d_frec.as<FiltRec>();
+    a.red = p.d_red.as<float>();
+    a.H = p.d_H.as<float2>();
+    a.tw = p.d_tw.as<float2>();
+    // the filter kernel writes view v at buffer index v - filt_view_offset
+    const size_t view_elems = size_t(p.rows_pv) * p.pitch;
+    a.filt = static_cast<char *>(p.d_filt.p) -
+             static_cast<ptrdiff_t>(size_t(filt_view_offset) * view_elems * storage_bytes(p.opts.storage));
+    a.pitch = p.pitch;
+    a.rows_pv = p.rows_pv;
+    a.storage = p.opts.storage;
+    return a;
+}
+
+BpLaunch bp_args(gct_plan &p, float *vol, uint32_t first, uint32_t n, bool accumulate, bool finish,
+                 uint32_t filt_view_offset)
+{
+    BpLaunch a{};
+    a.kind = p.geom.kind;
+    a.storage = p.opts.storage;
+    a.circular = p.circular;
+    a.setup_f64 = p.setup_f64;
+    a.nx = p.img.x.num_samples;
+    a.ny = p.img.y.num_samples;
+    a.nz = (p.img.dim == 3) ? p.img.z.num_samples : 1u;
+    a.nu = p.nu;
+    a.nv = p.nv;
+    a.first_view = first;
+    a.num_views = n;
+    a.pitch = p.pitch;
+    a.rows_pv = p.rows_pv;
+    const size_t view_elems = size_t(p.rows_pv) * p.pitch;
+    a.filt = static_cast<const char *>(p.d_filt.p) -
+             static_cast<ptrdiff_t>(size_t(filt_view_offset) * view_elems * storage_bytes(p.opts.storage));
+    a.rec = p.d_brec.as<BpRec>();
+    a.rec64 = p.d_brec64.as<BpRec64>();
+    a.vol = vol;
+    a.inv_dgamma = p.inv_dgamma;
+    a.scale = p.scale;
+    a.accumulate = accumulate;
+    a.finish = finish;
+    a.z_unroll = p.z_unroll;
+    return a;
+}
+
+void check_range(const gct_plan &p, uint32_t first, uint32_t n)
+{
+    if (first > p.V || n > p.V - first) fail(GCT_ERR_INVALID_ARGUMENT, "view range outside the scan");
+    if (n > p.filt_views) fail(GCT_ERR_INVALID_ARGUMENT, "view range larger than the plan's filtered buffer");
+}
+
+uint64_t plan_device_bytes(const gct_ct_geom &g, const gct_image_geom &img, const gct_recon_opts &o, uint32_t fviews)
+{
+    const uint64_t nu = g.det_row.num_samples, nv = is_1d(g.kind) ? 1u : g.det_col.num_samples;
+    const uint64_t pitch = (nu + 3u + 15u) & ~uint64_t(15);
+    const uint64_t V = g.num_proj;
+    const uint64_t L = next_pow2(uint32_t(2u * nu));
+    uint64_t b = uint64_t(fviews ? fviews : V) * (nv + 3u) * pitch * storage_bytes(o.storage);
+    b += V * (sizeof(BpRec) + sizeof(BpRec64) + sizeof(FiltRec) + sizeof(gct_proj_geom)) + V * nu * 4u + 2u * L * 8u;
+    (void)img;
+    return b;
+}
+
+uint64_t volume_bytes(const gct_image_geom &img)
+{
+    return uint64_t(img.x.num_samples) * img.y.num_samples * (img.dim == 3 ? img.z.num_samples : 1u) * sizeof(float);
+}
+
+}  // namespace
+}  // namespace gct
+
+// ---- C ABI ------------------------------------------------------------------
+
+extern "C" uint32_t gct_abi_version(void) { return GCT_AMD_ABI_VERSION; }
+
+extern "C" const char *gct_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" const char *gct_status_string(gct_status s)
+{
+    switch (s) {
+    case GCT_OK: return "ok";
+    case GCT_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case GCT_ERR_RUNTIME: return "runtime error";
+    case GCT_ERR_OUT_OF_MEMORY: return "out of memory";
+    case GCT_ERR_NO_DEVICE: return "no device";
+    case GCT_ERR_UNSUPPORTED: return "unsupported";
+    case GCT_ERR_REJECTED: return "rejected";
+    case GCT_ERR_NOT_READY: return "not ready";
+    }
+    return "unknown status";
+}
+
+extern "C" void gct_recon_opts_default(gct_recon_opts *o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->cut_off_freq = 1.0f;
+    o->window = GCT_WINDOW_NONE;
+    o->storage = GCT_STORE_F32;
+    o->setup_precision = GCT_SETUP_AUTO;
+    o->mem_mode = GCT_MEM_AUTO;
+    o->device = 0;
+}
+
+extern "C" int gct_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+static gct_recon_opts opts_or_default(const gct_recon_opts *o)
+{
+    gct_recon_opts d;
+    gct_recon_opts_default(&d);
+    return o ? *o : d;
+}
+
+extern "C" gct_status gct_plan_create(const gct_ct_geom *geom, const gct_image_geom *img, const gct_recon_opts *opts,
+                                      gct_plan **out)
+{
+    return guarded([&] {
+        if (!geom || !img || !out) fail(GCT_ERR_INVALID_ARGUMENT, "null argument");
+        if (gct_device_count() == 0) fail(GCT_ERR_NO_DEVICE, "no HIP device");
+        *out = make_plan(*geom, *img, opts_or_default(opts), 0).release();
+    });
+}
+
+extern "C" void gct_plan_destroy(gct_plan *p)
+{
+    if (!p) return;
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(p->device);
+    delete p;
+    (void)hipSetDevice(prev);
+}
+
+extern "C" gct_status gct_plan_get_info(const gct_plan *p, gct_plan_info *info)
+{
+    return guarded([&] {
+        if (!p || !info) fail(GCT_ERR_INVALID_ARGUMENT, "null argument");
+        std::memset(info, 0, sizeof(*info));
+        info->fft_len = p->L;
+        info->filt_pitch = p->pitch;
+        info->filt_rows_per_view = p->rows_pv;
+        info->z_unroll = p->z_unroll;
+        info->filt_bytes = p->d_filt.bytes;
+        info->device_bytes = p->d_filt.bytes + p->d_brec.bytes + p->d_brec64.bytes + p->d_frec.bytes + p->d_red.bytes +
+                             p->d_H.bytes + p->d_tw.bytes + p->d_views.bytes;
+        info->updates = uint64_t(p->img.x.num_samples) * p->img.y.num_samples *
+                        (p->img.dim == 3 ? p->img.z.num_samples : 1u) * p->V;
+        info->rot_ang_incr = p->rot.incr;
+        info->rot_ang_range = p->rot.range;
+        info->rot_dir = p->rot.dir;
+        info->setup_f64 = p->setup_f64;
+        info->circular = p->circular;
+        info->storage = p->opts.storage;
+    });
+}
+
+extern "C" gct_status gct_plan_filter(gct_plan *p, const float *proj, uint32_t first, uint32_t n, void *stream)
+{
+    return guarded([&] {
+        if (!p || !proj) fail(GCT_ERR_INVALID_ARGUMENT, "null argument");
+        check_range(*p, first, n);
+        DeviceGuard dg(p->device);
+        launch_filter(filter_args(*p, proj, 0, first, n, p->filt_views < p->V ? first : 0), static_cast<hipStream_t>(stream));
+    });
+}
+
+extern "C" gct_status gct_plan_backproject(gct_plan *p, float *vol, uint32_t first, uint32_t n, int accumulate,
+                                           int finish, void *stream)
+{
+    return guarded([&] {
+        if (!p || !vol) fail(GCT_ERR_INVALID_ARGUMENT, "null argument");
+        check_range(*p, first, n);
+        DeviceGuard dg(p->device);
+        launch_backproject(bp_args(*p, vol, first, n, accumulate != 0, finish != 0, p->filt_views < p->V ? first : 0),
+                           static_cast<hipStream_t>(stream));
+    });
+}
+
+extern "C" gct_status gct_plan_execute(gct_plan *p, const float *proj, float *vol, void *stream)
+{
+    return guarded([&] {
+        if (!p || !proj || !vol) fail(GCT_ERR_INVALID_ARGUMENT, "null argument");
+        if (p->filt_views < p->V) fail(GCT_ERR_INVALID_ARGUMENT, "plan holds a partial filtered buffer");
+        DeviceGuard dg(p->device);
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        launch_filter(filter_args(*p, proj, 0, 0, p->V, 0), s);
+        launch_backproject(bp_args(*p, vol, 0, p->V, false, true, 0), s);
+    });
+}
+
+extern "C" gct_status gct_plan_load_filtered(gct_plan *p, const float *filt, void *stream)
+{
+    return guarded([&] {
+        if (!p || !filt) fail(GCT_ERR_INVALID_ARGUMENT, "null argument");
+        DeviceGuard dg(p->device);
+        launch_pack_filtered(filt, p->d_filt.p, p->opts.storage, p->nu, p->nv, p->filt_views, p->pitch, p->rows_pv,
+                             static_cast<hipStream_t>(stream));
+    });
+}
+
+extern "C" gct_status gct_plan_store_filtered(gct_plan *p, float *filt, void *stream)
+{
+    return guarded([&] {
+        if (!p || !filt) fail(GCT_ERR_INVALID_ARGUMENT, "null argument");
+        DeviceGuard dg(p->device);
+        launch_unpack_filtered(p->d_filt.p, filt, p->opts.storage, p->nu, p->nv, p->filt_views, p->pitch, p->rows_pv,
+                               static_cast<hipStream_t>(stream));
+    });
+}
+
+extern "C" gct_status gct_plan_forward_project(gct_plan *p, const float *vol, float *proj, void *stream)
+{
+    return guarded([&] {
+        if (!p || !vol || !proj) fail(GCT_ERR_INVALID_ARGUMENT, "null argument");
+        DeviceGuard dg(p->device);
+        FpLaunch a{};
+        a.kind = p->geom.kind;
+        a.nx = p->img.x.num_samples;
+        a.ny = p->img.y.num_samples;
+        a.nz = (p->img.dim == 3) ? p->img.z.num_samples : 1u;
+        a.dx = p->img.x.sampling_width;
+        a.dy = p->img.y.sampling_width;
+        a.dz = (p->img.dim == 3) ? p->img.z.sampling_width : 1.0f;
+        a.x0 = static_cast<float>(double(p->img.mid_point[0]) - (double(a.nx) - 1.0) * a.dx / 2.0);
+        a.y0 = static_cast<float>(double(p->img.mid_point[1]) - (double(a.ny) - 1.0) * a.dy / 2.0);
+        a.z0 = (p->img.dim == 3) ? static_cast<float>(double(p->img.mid_point[2]) - (double(a.nz) - 1.0) * a.dz / 2.0) : 0.0f;
+        a.nu = p->nu;
+        a.nv = p->nv;
+        a.nviews = p->V;
+        a.du = p->geom.det_row.sampling_width;
+        a.dv = is_1d(p->geom.kind) ? 1.0f : p->geom.det_col.sampling_width;
+        a.views = p->d_views.as<gct_proj_geom>();
+        a.vol = vol;
+        a.proj = proj;
+        launch_forward_project(a, static_cast<hipStream_t>(stream));
+    });
+}
+
+extern "C" gct_status gct_required_memory(const gct_ct_geom *geom, const gct_image_geom *img,
+                                          const gct_recon_opts *opts, uint64_t *bytes)
+{
+    return guarded([&] {
+        if (!geom || !img || !bytes) fail(GCT_ERR_INVALID_ARGUMENT, "null argument");
+        validate_geom(*geom);
+        validate_image(*geom, *img);
+        const gct_recon_opts o = opts_or_default(opts);
+        const uint64_t nv = is_1d(geom->kind) ? 1u : geom->det_col.num_samples;
+        const uint64_t proj = uint64_t(geom->num_proj) * nv * geom->det_row.num_samples * sizeof(float);
+        *bytes = proj + volume_bytes(*img) + plan_device_bytes(*geom, *img, o, 0);
+    });
+}
+
+// reconFDK (fdk_axial.cuh:69-89): every view resident when the estimate is
+// below 75 % of device memory, otherwise views stream through in chunks.
+// Both paths add each voxel's views in scan order, so they give the same
+// volume bit for bit (the property fdk_axial_tests.cu:68-73 asserts).
+extern "C" gct_status gct_reconstruct(const gct_ct_geom *geom, const float *proj, const gct_image_geom *img, float *vol,
+                                      const gct_recon_opts *opts)
+{
+    return guarded([&] {
+        if (!geom || !proj || !img || !vol) fail(GCT_ERR_INVALID_ARGUMENT, "null argument");
+        if (gct_device_count() == 0) fail(GCT_ERR_NO_DEVICE, "no HIP device");
+        const gct_recon_opts o = opts_or_default(opts);
+        validate_geom(*geom);
+        validate_image(*geom, *img);
+        DeviceGuard dg(o.device);
+        size_t free_b = 0, total_b = 0;
+        hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        const uint64_t nv = is_1d(geom->kind) ? 1u : geom->det_col.num_samples;
+        const uint64_t view_bytes = nv * geom->det_row.num_samples * sizeof(float);
+        const uint64_t need = uint64_t(geom->num_proj) * view_bytes + volume_bytes(*img) + plan_device_bytes(*geom, *img, o, 0);
+        bool low = (o.mem_mode == GCT_MEM_LOW) ||
+                   (o.mem_mode == GCT_MEM_AUTO && double(need) / double(total_b) >= 0.75);
+        uint32_t chunk = geom->num_proj;
+        if (low) {
+            chunk = o.view_chunk ? o.view_chunk : 0u;
+            if (!chunk) {
+                // chunks of ~1/16 of free memory for staging and filtered views
+                const uint64_t per = view_bytes + (nv + 3u) * ((geom->det_row.num_samples + 18u) & ~15u) * storage_bytes(o.storage);
+                chunk = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(geom->num_proj, (free_b / 16) / per)));
+            }
+            chunk = std::min(chunk, geom->num_proj);
+        }
+        auto plan = make_plan(*geom, *img, o, chunk);
+        hipStream_t s = nullptr;
+        hip_check(hipStreamCreate(&s), "hipStreamCreate");
+        struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamDestroy(s); } } sg{s};
+        DevBuf dvol(volume_bytes(*img));
+        if (!low) {
+            DevBuf dproj(size_t(geom->num_proj) * view_bytes);
+            hip_check(hipMemcpyAsync(dproj.p, proj, dproj.bytes, hipMemcpyHostToDevice, s), "copy projections");
+            launch_filter(filter_args(*plan, dproj.as<float>(), 0, 0, plan->V, 0), s);
+            launch_backproject(bp_args(*plan, dvol.as<float>(), 0, plan->V, false, true, 0), s);
+            hip_check(hipMemcpyAsync(vol, dvol.p, dvol.bytes, hipMemcpyDeviceToHost, s), "copy volume");
+            hip_check(hipStreamSynchronize(s), "reconstruct");
+        } else {
+            // two staging buffers: the copy of chunk i+1 overlaps the work on chunk i
+            DevBuf stage[2] = {DevBuf(size_t(chunk) * view_bytes), DevBuf(size_t(chunk) * view_bytes)};
+            hipEvent_t done[2];
+            hip_check(hipEventCreateWithFlags(&done[0], hipEventDisableTiming), "event");
+            hip_check(hipEventCreateWithFlags(&done[1], hipEventDisableTiming), "event");
+            hipStream_t cs = nullptr;
+            hip_check(hipStreamCreate(&cs), "hipStreamCreate");
+            StreamGuard csg{cs};
+            hipEvent_t copied[2];
+            hip_check(hipEventCreateWithFlags(&copied[0], hipEventDisableTiming), "event");
+            hip_check(hipEventCreateWithFlags(&copied[1], hipEventDisableTiming), "event");
+            bool used[2] = {false, false};
+            uint32_t idx = 0;
+            for (uint32_t first = 0; first < plan->V; first += chunk, idx ^= 1u) {
+                const uint32_t n = std::min(chunk, plan->V - first);
+                if (used[idx]) hip_check(hipStreamWaitEvent(cs, done[idx], 0), "wait");
+                hip_check(hipMemcpyAsync(stage[idx].p, proj + size_t(first) * view_bytes / sizeof(float), size_t(n) * view_bytes,
+                                         hipMemcpyHostToDevice, cs),
+                          "copy projection chunk");
+                hip_check(hipEventRecord(copied[idx], cs), "record");
+                hip_check(hipStreamWaitEvent(s, copied[idx], 0), "wait");
+                launch_filter(filter_args(*plan, stage[idx].as<float>(), first, first, n, first), s);
+                launch_backproject(bp_args(*plan, dvol.as<float>(), first, n, first > 0, first + n >= plan->V, first), s);
+                hip_check(hipEventRecord(done[idx], s), "record");
+                used[idx] = true;
+            }
+            hip_check(hipMemcpyAsync(vol, dvol.p, dvol.bytes, hipMemcpyDeviceToHost, s), "copy volume");
+            hip_check(hipStreamSynchronize(s), "reconstruct");
+            for (int i = 0; i < 2; ++i) { (void)hipEventDestroy(done[i]); (void)hipEventDestroy(copied[i]); }
+        }
+    });
+}
+
+// ---- autotuner ----------------------------------------------------------------
+
+extern "C" gct_status gct_plan_autotune(gct_plan *p, float *vol, uint32_t *best_zu, float *best_ms)
+{
+    return guarded([&] {
+        if (!p || !vol) fail(GCT_ERR_INVALID_ARGUMENT, "null argument");
+        if (p->filt_views < p->V) fail(GCT_ERR_INVALID_ARGUMENT, "plan holds a partial filtered buffer");
+        DeviceGuard dg(p->device);
+        hipStream_t s = nullptr;
+        hip_check(hipStreamCreate(&s), "hipStreamCreate");
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "event");
+        hip_check(hipEventCreate(&e1), "event");
+        const uint32_t cands[3] = {4u, 8u, 16u};
+        float best = 1e30f;
+        uint32_t bz = p->z_unroll;
+        const uint32_t saved = p->z_unroll;
+        for (uint32_t zu : cands) {
+            p->z_unroll = zu;
+            const BpLaunch a = bp_args(*p, vol, 0, p->V, false, true, 0);
+            launch_backproject(a, s);   // warm-up
+            float tot = 0.0f;
+            const int reps = 3;
+            for (int r = 0; r < reps; ++r) {
+                hip_check(hipEventRecord(e0, s), "record");
+                launch_backproject(a, s);
+                hip_check(hipEventRecord(e1, s), "record");
+                hip_check(hipEventSynchronize(e1), "sync");
+                float ms = 0.0f;
+                hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+                tot += ms;
+            }
+            if (tot / reps < best) { best = tot / reps; bz = zu; }
+        }
+        p->z_unroll = bz;
+        (void)saved;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tune[{p->geom.kind, p->img.x.num_samples, p->img.y.num_samples, p->img.dim == 3 ? p->img.z.num_samples : 1u,
+                    p->V, p->nu, p->nv, p->opts.storage, int(p->circular), int(p->setup_f64)}] = Tuning{bz};
+        }
+        if (best_zu) *best_zu = bz;
+        if (best_ms) *best_ms = best;
+    });
+}
+
+// ---- batches ------------------------------------------------------------------
+
+struct gct_batch {
+    int device = 0;
+    std::vector<gct_plan *> plans;    // creation order
+    std::vector<uint32_t> order;      // work order: largest scans first
+    std::vector<BatchScan> scans;
+    uint32_t total_tiles = 0;
+    DevBuf d_scans, d_counter;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<const float *> g_proj;
+    std::vector<float *> g_vol;
+    hipStream_t g_stream = nullptr;
+    ~gct_batch()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+extern "C" gct_status gct_batch_create(gct_plan *const *plans, uint32_t n, gct_batch **out)
+{
+    return guarded([&] {
+        if (!plans || !n || !out) fail(GCT_ERR_INVALID_ARGUMENT, "empty batch");
+        auto b = std::make_unique<gct_batch>();
+        b->device = plans[0] ? plans[0]->device : 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!plans[i]) fail(GCT_ERR_INVALID_ARGUMENT, "null plan");
+            if (plans[i]->device != b->device) fail(GCT_ERR_INVALID_ARGUMENT, "plans on different devices");
+            if (plans[i]->filt_views < plans[i]->V) fail(GCT_ERR_INVALID_ARGUMENT, "plan holds a partial filtered buffer");
+            if (plans[i]->setup_f64) fail(GCT_ERR_UNSUPPORTED, "batched backprojection uses the FP32 setup");
+            b->plans.push_back(plans[i]);
+        }
+        b->order.resize(n);
+        for (uint32_t i = 0; i < n; ++i) b->order[i] = i;
+        auto cost = [&](uint32_t i) {
+            const gct_plan *p = b->plans[i];
+            return uint64_t(p->img.x.num_samples) * p->img.y.num_samples * (p->img.dim == 3 ? p->img.z.num_samples : 1u) * p->V;
+        };
+        std::stable_sort(b->order.begin(), b->order.end(), [&](uint32_t a, uint32_t c) { return cost(a) > cost(c); });
+        DeviceGuard dg(b->device);
+        b->d_counter = DevBuf(sizeof(uint32_t));
+        *out = b.release();
+    });
+}
+
+extern "C" void gct_batch_destroy(gct_batch *b)
+{
+    if (!b) return;
+    int prev = 0;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(b->device);
+    delete b;
+    (void)hipSetDevice(prev);
+}
+
+namespace gct {
+namespace {
+
+// Builds and uploads the work table (synchronous; never inside a capture).
+void batch_prepare(gct_batch &b, const float *const *proj, float *const *vol)
+{
+    (void)proj;
+    b.scans.clear();
+    uint32_t tile = 0;
+    for (uint32_t oi : b.order) {
+        gct_plan &p = *b.plans[oi];
+        if (!vol[oi]) fail(GCT_ERR_INVALID_ARGUMENT, "null volume");
+        BatchScan sc{};
+        sc.bp = bp_args(p, vol[oi], 0, p.V, false, true, 0);
+        sc.bp.z_unroll = 8;
+        sc.tiles_x = (p.img.x.num_samples + 15u) / 16u;
+        sc.tiles_y = (p.img.y.num_samples + 15u) / 16u;
+        sc.tiles_z = is_1d(p.geom.kind) ? 1u : (p.img.z.num_samples + 7u) / 8u;
+        sc.tile_begin = tile;
+        tile += sc.tiles_x * sc.tiles_y * sc.tiles_z;
+        b.scans.push_back(sc);
+    }
+    b.total_tiles = tile;
+    const size_t bytes = b.scans.size() * sizeof(BatchScan);
+    if (b.d_scans.bytes != bytes) b.d_scans = DevBuf(bytes);
+    hip_check(hipMemcpy(b.d_scans.p, b.scans.data(), bytes, hipMemcpyHostToDevice), "upload batch table");
+}
+
+// Stream work only: filters of every scan, then one persistent backprojection.
+void batch_record(gct_batch &b, const float *const *proj, hipStream_t s)
+{
+    for (uint32_t oi : b.order) {
+        gct_plan &p = *b.plans[oi];
+        if (!proj[oi]) fail(GCT_ERR_INVALID_ARGUMENT, "null projections");
+        launch_filter(filter_args(p, proj[oi], 0, 0, p.V, 0), s);
+    }
+    launch_backproject_batch(b.d_scans.as<BatchScan>(), uint32_t(b.scans.size()), b.total_tiles,
+                             b.d_counter.as<uint32_t>(), s);
+}
+
+}  // namespace
+}  // namespace gct
+
+extern "C" gct_status gct_batch_execute(gct_batch *b, const float *const *proj, float *const *vol, int use_graph,
+                                        void *stream)
+{
+    return guarded([&] {
+        if (!b || !proj || !vol) fail(GCT_ERR_INVALID_ARGUMENT, "null argument");
+        DeviceGuard dg(b->device);
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const size_t n = b->plans.size();
+        std::vector<const float *> P(proj, proj + n);
+        std::vector<float *> Vv(vol, vol + n);
+        if (!use_graph) {
+            batch_prepare(*b, proj, vol);
+            batch_record(*b, proj, s);
+            return;
+        }
+        const bool same = b->exec && b->g_stream == s && b->g_proj == P && b->g_vol == Vv;
+        if (!same) {
+            if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+            if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+            hip_check(hipStreamSynchronize(s), "sync before capture");
+            batch_prepare(*b, proj, vol);
+            hipStream_t cap = nullptr;
+            hip_check(hipStreamCreate(&cap), "hipStreamCreate");
+            hip_check(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "begin capture");
+            try {
+                batch_record(*b, proj, cap);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(cap, &g);
+                if (g) (void)hipGraphDestroy(g);
+                (void)hipStreamDestroy(cap);
+                throw;
+            }
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(cap, &g);
+            (void)hipStreamDestroy(cap);
+            hip_check(e, "end capture");
+            b->graph = g;
+            hip_check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph instantiate");
+            b->g_proj = P;
+            b->g_vol = Vv;
+            b->g_stream = s;
+        }
+        hip_check(hipGraphLaunch(b->exec, s), "graph launch");
+    });
+}

@@ -1,0 +1,146 @@
+"""Synthetic workloads: the scan geometries, image grids and seeded phantoms
+the benchmark and the tests reconstruct.
+
+Every workload is generated here from its stated parameters (no data files):
+analytic line integrals of random ellipses/ellipsoids (FP64, cast to FP32),
+optionally with Poisson noise.  Densities of 3-D phantoms are drawn in cm^-1
+and stored in mm^-1 (x 0.1) so that photon counts at I0 = 1e5 stay meaningful.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import (CONE_PARALLEL, CYLINDRICAL, FAN_ARC_1D, FAN_LINE_1D, FLAT, PARALLEL_1D, PARALLEL_2D, Image, Scan,
+               WINDOW_NONE, WINDOW_SHEPP_LOGAN, circular_scan, is_1d, phantom_project, phantom_random)
+
+
+@dataclass
+class Workload:
+    name: str
+    scan: Scan
+    img: Image
+    phantom: np.ndarray
+    i0: float = 0.0
+    seed: int = 0
+    window: int = WINDOW_NONE
+    storage_forms: tuple = ("f32",)
+    notes: dict = field(default_factory=dict)
+
+    def projections(self, nthreads: int = 0) -> np.ndarray:
+        return phantom_project(self.scan, self.phantom, self.i0, self.seed + 7, nthreads)
+
+    @property
+    def updates(self) -> int:
+        return self.img.nvox * self.scan.num_proj
+
+
+def parallel_1d_256() -> Workload:
+    """2-D parallel-beam FDK: 256^2 at 0.5 mm, 360 views over 180 deg, 367 bins
+    at 0.5 mm (centre 183), Ram-Lak; 10 seeded ellipses (centres within 0.8 R,
+    semi-axes U(0.05, 0.4) R, rotation U(0, pi), attenuation U(-0.3, 1.0))."""
+    seed = 1001
+    scan = circular_scan(PARALLEL_1D, 367, 0.5, 360, math.pi / 360.0, det_row_center=183.0)
+    img = Image(256, 256, 0.5, 0.5)
+    ph = phantom_random(seed, 10, 2, 64.0, (0.05, 0.4), (-0.3, 1.0))   # FOV radius 64 mm
+    return Workload("parallel_1d_256", scan, img, ph, 0.0, seed, WINDOW_NONE)
+
+
+def flat_cone_256(window: int = WINDOW_SHEPP_LOGAN, noise: bool = True) -> Workload:
+    """Full-scan flat-detector cone beam: SOD 500, SDD 1000 mm, 512 views over
+    360 deg, 512 x 384 detector at 0.4 mm (centre 255.5, 191.5), 256^3 volume
+    at 0.25 mm; 12 seeded ellipsoids, Poisson noise at I0 = 1e5."""
+    seed = 1002
+    scan = circular_scan(FLAT, 512, 0.4, 512, 2.0 * math.pi / 512, nv=384, dv=0.4, det_row_center=255.5,
+                         det_col_center=191.5, src_det_dist=1000.0, src_iso_dist=500.0)
+    img = Image(256, 256, 0.25, 0.25, 256, 0.25)
+    ph = phantom_random(seed, 12, 3, 32.0, (0.05, 0.35), (-0.02, 0.1))
+    return Workload("flat_cone_256", scan, img, ph, 1e5 if noise else 0.0, seed, window, ("f32",))
+
+
+def carm_short_scan_512() -> Workload:
+    """Short-scan C-arm cone beam with Parker weighting: SOD 750, SDD 1200 mm,
+    248 views over 200 deg, 616 x 480 flat detector at 0.616 mm (centre 307.5,
+    239.5), 512^3 volume at 0.3 mm; FP32/FP16/BF16 filtered-projection storage."""
+    seed = 1003
+    scan = circular_scan(FLAT, 616, 0.616, 248, math.radians(200.0) / 248, nv=480, dv=0.616, det_row_center=307.5,
+                         det_col_center=239.5, src_det_dist=1200.0, src_iso_dist=750.0)
+    img = Image(512, 512, 0.3, 0.3, 512, 0.3)
+    ph = phantom_random(seed, 12, 3, 76.8, (0.05, 0.35), (-0.02, 0.1))
+    return Workload("carm_short_scan_512", scan, img, ph, 1e5, seed, WINDOW_NONE, ("f32", "f16", "bf16"))
+
+
+def fan_arc_slice(rng: np.random.Generator, idx: int) -> Workload:
+    """One slice of the uneven batch: arc detector, channels U{300..900},
+    views U{360..1200} over 360 deg, image U{256..768}^2, SOD 570, SDD 1040 mm.
+    The fan spans 0.957 rad (a 500 mm field of view with 5 % margin); the
+    pixel is 500 mm / N."""
+    nch = int(rng.integers(300, 901))
+    nview = int(rng.integers(360, 1201))
+    n = int(rng.integers(256, 769))
+    dg = 0.957 / nch
+    scan = circular_scan(FAN_ARC_1D, nch, dg, nview, 2.0 * math.pi / nview, det_row_center=(nch - 1) / 2.0,
+                         src_det_dist=1040.0, src_iso_dist=570.0)
+    img = Image(n, n, 500.0 / n, 500.0 / n)
+    ph = phantom_random(5000 + idx, 10, 2, 250.0, (0.05, 0.35), (-0.02, 0.1))
+    return Workload(f"fan_arc_{idx}", scan, img, ph, 0.0, 5000 + idx)
+
+
+def cylindrical_slab(idx: int) -> Workload:
+    """One MSCT slab of the uneven batch: cylindrical detector 672 channels x 64
+    rows (row pitch 1.095 mm, 0.6 mm at iso), 1152 views, 512 x 512 x 64 volume
+    at 0.977 x 0.977 x 0.6 mm, SOD 570, SDD 1040 mm."""
+    dg = 0.957 / 672
+    scan = circular_scan(CYLINDRICAL, 672, dg, 1152, 2.0 * math.pi / 1152, nv=64, dv=1.095,
+                         det_row_center=335.5, det_col_center=31.5, src_det_dist=1040.0, src_iso_dist=570.0)
+    img = Image(512, 512, 0.977, 0.977, 64, 0.6)
+    ph = phantom_random(6000 + idx, 12, 3, 250.0, (0.05, 0.35), (-0.02, 0.1))
+    return Workload(f"cyl_slab_{idx}", scan, img, ph, 0.0, 6000 + idx)
+
+
+def uneven_batch(num_slices: int = 64, num_slabs: int = 16, seed: int = 1004) -> list[Workload]:
+    rng = np.random.default_rng(seed)
+    out = [fan_arc_slice(rng, i) for i in range(num_slices)]
+    out += [cylindrical_slab(i) for i in range(num_slabs)]
+    return out
+
+
+def small_case(kind: int, n: int = 64, views: int = 120, nu: int = 96, nv: int = 80, short_scan: bool = False,
+               direction: int = 1, seed: int = 42) -> Workload:
+    """Small instance of every geometry for parity tests against the oracle."""
+    span = math.radians(200.0) if short_scan else 2.0 * math.pi
+    if kind in (PARALLEL_1D, PARALLEL_2D, CONE_PARALLEL) and not short_scan:
+        span = math.pi
+    incr = span / views
+    fov = 0.5 * n * 0.5   # radius in mm with 0.5 mm pixels
+    sid, sdd = 200.0, 400.0
+    if kind == PARALLEL_1D:
+        scan = circular_scan(kind, nu, 0.6, views, incr, det_row_center=(nu - 1) / 2.0, direction=direction)
+    elif kind == FAN_LINE_1D:
+        scan = circular_scan(kind, nu, 1.0, views, incr, det_row_center=(nu - 1) / 2.0 + 0.25, src_det_dist=sdd,
+                             src_iso_dist=sid, direction=direction)
+    elif kind == FAN_ARC_1D:
+        scan = circular_scan(kind, nu, 0.5 / nu * 1.6, views, incr, det_row_center=(nu - 1) / 2.0 - 0.3,
+                             src_det_dist=sdd, src_iso_dist=sid, direction=direction)
+    elif kind == PARALLEL_2D:
+        scan = circular_scan(kind, nu, 0.6, views, incr, nv=nv, dv=0.6, det_row_center=(nu - 1) / 2.0,
+                             det_col_center=(nv - 1) / 2.0 + 0.5, direction=direction)
+    elif kind == CONE_PARALLEL:
+        scan = circular_scan(kind, nu, 0.6, views, incr, nv=nv, dv=0.9, det_row_center=(nu - 1) / 2.0,
+                             det_col_center=(nv - 1) / 2.0, src_det_dist=sdd, src_iso_dist=sid, direction=direction)
+    elif kind == CYLINDRICAL:
+        scan = circular_scan(kind, nu, 0.5 / nu * 1.6, views, incr, nv=nv, dv=1.0, det_row_center=(nu - 1) / 2.0,
+                             det_col_center=(nv - 1) / 2.0, src_det_dist=sdd, src_iso_dist=sid, direction=direction)
+    else:
+        scan = circular_scan(kind, nu, 1.0, views, incr, nv=nv, dv=1.0, det_row_center=(nu - 1) / 2.0 + 0.5,
+                             det_col_center=(nv - 1) / 2.0 - 0.25, src_det_dist=sdd, src_iso_dist=sid,
+                             direction=direction)
+    if is_1d(kind):
+        img = Image(n, n, 0.5, 0.5, mid=(0.3, -0.2))
+        ph = phantom_random(seed, 8, 2, fov, (0.05, 0.35), (-0.02, 0.1))
+    else:
+        img = Image(n, n, 0.5, 0.5, n, 0.5, mid=(0.3, -0.2, 0.1))
+        ph = phantom_random(seed, 8, 3, fov, (0.05, 0.35), (-0.02, 0.1))
+    return Workload(f"small_{kind}", scan, img, ph, 0.0, seed)
