@@ -30,6 +30,8 @@ STORE_F32, STORE_F16, STORE_BF16 = range(3)
 SETUP_AUTO, SETUP_F32, SETUP_F64 = range(3)
 # gct_mem_mode
 MEM_AUTO, MEM_HIGH, MEM_LOW = range(3)
+# gct_bp_kernel
+BP_AUTO, BP_DIRECT, BP_LDS = range(3)
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "runtime error", 3: "out of memory", 4: "no device",
           5: "unsupported", 6: "rejected", 7: "not ready"}
@@ -69,7 +71,7 @@ class RotationGeom(C.Structure):
 class ReconOpts(C.Structure):
     _fields_ = [("cut_off_freq", C.c_float), ("window", C.c_int32), ("storage", C.c_int32),
                 ("setup_precision", C.c_int32), ("mem_mode", C.c_int32), ("device", C.c_int32),
-                ("z_unroll", C.c_uint32), ("view_chunk", C.c_uint32)]
+                ("z_unroll", C.c_uint32), ("view_chunk", C.c_uint32), ("bp_kernel", C.c_int32)]
 
 
 class CircularParams(C.Structure):
@@ -82,7 +84,8 @@ class PlanInfo(C.Structure):
     _fields_ = [("fft_len", C.c_uint32), ("filt_pitch", C.c_uint32), ("filt_rows_per_view", C.c_uint32),
                 ("z_unroll", C.c_uint32), ("filt_bytes", C.c_uint64), ("device_bytes", C.c_uint64),
                 ("updates", C.c_uint64), ("rot_ang_incr", C.c_float), ("rot_ang_range", C.c_float),
-                ("rot_dir", C.c_int32), ("setup_f64", C.c_int32), ("circular", C.c_int32), ("storage", C.c_int32)]
+                ("rot_dir", C.c_int32), ("setup_f64", C.c_int32), ("circular", C.c_int32), ("storage", C.c_int32),
+                ("bp_kernel", C.c_int32)]
 
 
 class ServerOpts(C.Structure):

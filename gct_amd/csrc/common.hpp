@@ -138,6 +138,7 @@ struct BpLaunch {
     float scale;                  // applied when finish
     bool accumulate, finish;
     uint32_t z_unroll;
+    int variant;                  // GCT_BP_DIRECT or GCT_BP_LDS
 };
 
 struct FpLaunch {

@@ -11,6 +11,7 @@
 //   (nd_array_dev.cuh:58-99).
 #include "common.hpp"
 
+#include <algorithm>
 #include <cstdint>
 
 namespace gct {
@@ -288,7 +289,7 @@ __device__ inline float linear(const typename Stor<S>::T *__restrict__ p, uint32
 
 // Per (thread, view) setup of a voxel column, shared by all z voxels.
 struct ColSetup {
-    Sample cs;
+    float col;     // column coordinate (unclamped)
     float w;       // weight (constant along z unless cylindrical)
     float row0;    // row coordinate of the first z voxel
     float drow;    // row step per z voxel
@@ -304,7 +305,7 @@ __device__ inline ColSetup setup_column(const BpLaunch &a, uint32_t v, float fx,
     if constexpr (KIND == GCT_PARALLEL_2D || KIND == GCT_CONE_PARALLEL) {
         const BpRec &r = a.rec[v];
         const float col = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
-        s.cs = col_sample(col, nuf);
+        s.col = col;
         s.w = 1.0f;
         s.row0 = fmaf(r.v[3], fz, r.v[0]);
         s.drow = r.v[3];
@@ -316,7 +317,7 @@ __device__ inline ColSetup setup_column(const BpLaunch &a, uint32_t v, float fx,
             const double nu = r.u[0] + r.u[1] * X + r.u[2] * Y + r.u[3] * Z;
             const double nv = r.v[0] + r.v[1] * X + r.v[2] * Y + r.v[3] * Z;
             const double rd = 1.0 / nw;
-            s.cs = col_sample(static_cast<float>(r.k[0] + nu * rd), nuf);
+            s.col = static_cast<float>(r.k[0] + nu * rd);
             s.w = static_cast<float>(rd * rd);
             s.row0 = static_cast<float>(r.k[1] + nv * rd);
             s.drow = static_cast<float>(r.v[3] * rd);
@@ -326,7 +327,7 @@ __device__ inline ColSetup setup_column(const BpLaunch &a, uint32_t v, float fx,
             const float nu = fmaf(r.u[3], fz, fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0])));
             const float nv = fmaf(r.v[3], fz, fmaf(r.v[2], fy, fmaf(r.v[1], fx, r.v[0])));
             const float rd = 1.0f / nw;
-            s.cs = col_sample(fmaf(nu, rd, r.k[0]), nuf);
+            s.col = fmaf(nu, rd, r.k[0]);
             s.w = rd * rd;
             s.row0 = fmaf(nv, rd, r.k[1]);
             s.drow = r.v[3] * rd;
@@ -340,7 +341,7 @@ __device__ inline ColSetup setup_column(const BpLaunch &a, uint32_t v, float fx,
             const double nv = r.v[0] + r.v[1] * X + r.v[2] * Y + r.v[3] * Z;
             const double rho2 = nu * nu + nw * nw;
             const double rr = 1.0 / sqrt(rho2);
-            s.cs = col_sample(static_cast<float>(r.k[0] + atan2(nu, nw) * static_cast<double>(a.inv_dgamma)), nuf);
+            s.col = static_cast<float>(r.k[0] + atan2(nu, nw) * static_cast<double>(a.inv_dgamma));
             s.row0 = static_cast<float>(r.k[1] + r.k[2] * nv * rr);
             s.drow = static_cast<float>(r.k[2] * r.v[3] * rr);
             s.rho2 = static_cast<float>(rho2);
@@ -354,7 +355,7 @@ __device__ inline ColSetup setup_column(const BpLaunch &a, uint32_t v, float fx,
             const float nv = fmaf(r.v[3], fz, fmaf(r.v[2], fy, fmaf(r.v[1], fx, r.v[0])));
             const float rho2 = fmaf(nu, nu, nw * nw);
             const float rr = 1.0f / sqrtf(rho2);
-            s.cs = col_sample(fmaf(atan2f(nu, nw), a.inv_dgamma, r.k[0]), nuf);
+            s.col = fmaf(atan2f(nu, nw), a.inv_dgamma, r.k[0]);
             s.row0 = fmaf(r.k[2] * nv, rr, r.k[1]);
             s.drow = r.k[2] * r.v[3] * rr;
             s.rho2 = rho2;
@@ -403,10 +404,11 @@ __device__ inline void bp_tile3d(const BpLaunch &a, uint32_t bx, uint32_t by, ui
         const T *p = base + static_cast<size_t>(v) * view_elems;
         if constexpr (column_invariant<KIND, CIRC>()) {
             const ColSetup s = setup_column<KIND, F64>(a, v, fx, fy, fz, nuf);
+            const Sample cs = col_sample(s.col, nuf);
 #pragma unroll
             for (int k = 0; k < ZU; ++k) {
                 const float row = fmaf(static_cast<float>(k), s.drow, s.row0);
-                const float val = bilinear<S>(p, a.pitch, s.cs, row, nvf);
+                const float val = bilinear<S>(p, a.pitch, cs, row, nvf);
                 float w = s.w;
                 if constexpr (KIND == GCT_CYLINDRICAL) {
                     const float nvk = fmaf(static_cast<float>(k), s.dnv, s.nv0);
@@ -418,7 +420,7 @@ __device__ inline void bp_tile3d(const BpLaunch &a, uint32_t bx, uint32_t by, ui
 #pragma unroll
             for (int k = 0; k < ZU; ++k) {
                 const ColSetup s = setup_column<KIND, F64>(a, v, fx, fy, fz + static_cast<float>(k), nuf);
-                const float val = bilinear<S>(p, a.pitch, s.cs, s.row0, nvf);
+                const float val = bilinear<S>(p, a.pitch, col_sample(s.col, nuf), s.row0, nvf);
                 float w = s.w;
                 if constexpr (KIND == GCT_CYLINDRICAL) w = s.w / fmaf(s.nv0, s.nv0, s.rho2);
                 acc[k] = fmaf(val, w, acc[k]);
@@ -496,6 +498,215 @@ __global__ __launch_bounds__(64) void bp2d_kernel(BpLaunch a)
     const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t bx = t % nbx, by = t / nbx;
     bp_tile2d<KIND, S>(a, bx * 8u + (threadIdx.x & 7u), by * 8u + (threadIdx.x >> 3));
+}
+
+// ---- LDS-staged backprojection ----------------------------------------------
+//
+// A workgroup owns a 16 x 16 x ZU voxel tile.  For each block of VB views it
+// projects the tile's 8 corner voxels, takes the bounding box of the detector
+// pixels they reach (plus one pixel of margin), copies those padded rows from
+// HBM into LDS (FP16/BF16 storage is widened to FP32 on the way), and every
+// thread then interpolates from LDS with two ds_read2 per update.  Coordinates
+// are clamped to the staged box: the box never extends past the zero padding
+// of the filtered-projection buffer, so the clamp also reproduces the
+// border-zero texture, and it keeps every LDS read inside the box.  A view
+// whose box does not fit the staging area is read from HBM directly.
+
+constexpr int kTP = 64;   // staged columns per row (LDS pitch)
+
+struct Box {
+    int clo, rlo, ncols, nrows;   // ncols == 0: the view is read from HBM
+};
+
+template <int KIND, int ZU>
+__device__ inline void tile_boxes(const BpLaunch &a, uint32_t bx, uint32_t by, uint32_t bz, uint32_t vb0,
+                                  uint32_t nvb, int TR, Box *boxes, uint32_t tid)
+{
+    if (tid >= 8u * nvb) return;
+    const uint32_t j = tid >> 3, corner = tid & 7u;
+    const uint32_t cx = min(bx * 16u + ((corner & 1u) ? 15u : 0u), a.nx - 1u);
+    const uint32_t cy = min(by * 16u + ((corner & 2u) ? 15u : 0u), a.ny - 1u);
+    const uint32_t cz = min(bz * ZU + ((corner & 4u) ? static_cast<uint32_t>(ZU - 1) : 0u), a.nz - 1u);
+    const float nuf = static_cast<float>(a.nu);
+    const ColSetup s = setup_column<KIND, false>(a, vb0 + j, static_cast<float>(cx), static_cast<float>(cy),
+                                                 static_cast<float>(cz), nuf);
+    float cmin = s.col, cmax = s.col, rmin = s.row0, rmax = s.row0;
+#pragma unroll
+    for (int off = 1; off < 8; off <<= 1) {
+        cmin = fminf(cmin, __shfl_xor(cmin, off));
+        cmax = fmaxf(cmax, __shfl_xor(cmax, off));
+        rmin = fminf(rmin, __shfl_xor(rmin, off));
+        rmax = fmaxf(rmax, __shfl_xor(rmax, off));
+    }
+    if (corner != 0u) return;
+    Box b{0, 0, 0, 0};
+    const bool finite = cmin > -1.0e6f && cmax < 1.0e6f && rmin > -1.0e6f && rmax < 1.0e6f;
+    if (finite) {
+        const int clo = max(static_cast<int>(floorf(cmin)) - 1, -1);
+        const int chi = min(static_cast<int>(floorf(cmax)) + 2, static_cast<int>(a.nu) + 1);
+        const int rlo = max(static_cast<int>(floorf(rmin)) - 1, -1);
+        const int rhi = min(static_cast<int>(floorf(rmax)) + 2, static_cast<int>(a.nv) + 1);
+        const int nc = chi - clo + 1, nr = rhi - rlo + 1;
+        if (nc >= 2 && nr >= 2 && nc <= kTP && nr <= TR) b = Box{clo, rlo, nc, nr};
+    }
+    boxes[j] = b;
+}
+
+template <int KIND, int S, bool F64, int ZU, int TR, int VB>
+__global__ __launch_bounds__(256) void bp3d_lds_kernel(BpLaunch a)
+{
+    using T = typename Stor<S>::T;
+    __shared__ float tile[VB * TR * kTP];
+    __shared__ Box boxes[VB];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nbx = (a.nx + 15u) / 16u, nby = (a.ny + 15u) / 16u;
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t bx = t % nbx, rest = t / nbx;
+    const uint32_t by = rest % nby, bz = rest / nby;
+    const uint32_t ix = bx * 16u + (tid & 15u), iy = by * 16u + (tid >> 4), iz0 = bz * ZU;
+    const bool active = ix < a.nx && iy < a.ny;
+    const uint32_t zn = min(static_cast<uint32_t>(ZU), a.nz - iz0);
+    const size_t slice = static_cast<size_t>(a.nx) * a.ny;
+    float *vp = a.vol + static_cast<size_t>(iz0) * slice + static_cast<size_t>(active ? iy : 0u) * a.nx + (active ? ix : 0u);
+
+    float acc[ZU];
+#pragma unroll
+    for (int k = 0; k < ZU; ++k)
+        acc[k] = (active && a.accumulate && static_cast<uint32_t>(k) < zn) ? vp[k * slice] : 0.0f;
+
+    const float fx = static_cast<float>(ix), fy = static_cast<float>(iy), fz = static_cast<float>(iz0);
+    const float nuf = static_cast<float>(a.nu), nvf = static_cast<float>(a.nv);
+    const size_t view_elems = static_cast<size_t>(a.rows_pv) * a.pitch;
+    const T *base = static_cast<const T *>(a.filt);
+    const uint32_t vend = a.first_view + a.num_views;
+
+    for (uint32_t vb0 = a.first_view; vb0 < vend; vb0 += VB) {
+        const uint32_t nvb = min(static_cast<uint32_t>(VB), vend - vb0);
+        tile_boxes<KIND, ZU>(a, bx, by, bz, vb0, nvb, TR, boxes, tid);
+        __syncthreads();
+        for (uint32_t j = 0; j < nvb; ++j) {
+            const Box b = boxes[j];
+            if (b.ncols == 0) continue;
+            const T *p = base + static_cast<size_t>(vb0 + j) * view_elems;
+            float *tl = tile + j * (TR * kTP);
+            const int c = static_cast<int>(tid & 63u);
+            if (c < b.ncols) {
+                for (int r = static_cast<int>(tid >> 6); r < b.nrows; r += 4)
+                    tl[r * kTP + c] = Stor<S>::ld(p, static_cast<uint32_t>(b.rlo + r + 1) * a.pitch +
+                                                          static_cast<uint32_t>(b.clo + c + 1));
+            }
+        }
+        __syncthreads();
+        if (active) {
+            for (uint32_t j = 0; j < nvb; ++j) {
+                const uint32_t v = vb0 + j;
+                const ColSetup s = setup_column<KIND, F64>(a, v, fx, fy, fz, nuf);
+                const Box b = boxes[j];
+                if (b.ncols != 0) {
+                    const float *tl = tile + j * (TR * kTP);
+                    const float cl = fminf(fmaxf(s.col, static_cast<float>(b.clo)), static_cast<float>(b.clo + b.ncols - 2));
+                    const float cf = floorf(cl);
+                    const float alpha = cl - cf;
+                    const int ci = static_cast<int>(cf) - b.clo;
+                    const float rlo = static_cast<float>(b.rlo), rhi = static_cast<float>(b.rlo + b.nrows - 2);
+#pragma unroll
+                    for (int k = 0; k < ZU; ++k) {
+                        const float row = fminf(fmaxf(fmaf(static_cast<float>(k), s.drow, s.row0), rlo), rhi);
+                        const float rf = floorf(row);
+                        const float beta = row - rf;
+                        const int idx = (static_cast<int>(rf) - b.rlo) * kTP + ci;
+                        const float t0 = tl[idx], t1 = tl[idx + 1];
+                        const float b0 = tl[idx + kTP], b1 = tl[idx + kTP + 1];
+                        const float top = fmaf(alpha, t1 - t0, t0);
+                        const float bot = fmaf(alpha, b1 - b0, b0);
+                        const float val = fmaf(beta, bot - top, top);
+                        float w = s.w;
+                        if constexpr (KIND == GCT_CYLINDRICAL) {
+                            const float nvk = fmaf(static_cast<float>(k), s.dnv, s.nv0);
+                            w = s.w / fmaf(nvk, nvk, s.rho2);
+                        }
+                        acc[k] = fmaf(val, w, acc[k]);
+                    }
+                } else {
+                    const T *p = base + static_cast<size_t>(v) * view_elems;
+                    const Sample cs = col_sample(s.col, nuf);
+#pragma unroll
+                    for (int k = 0; k < ZU; ++k) {
+                        const float row = fmaf(static_cast<float>(k), s.drow, s.row0);
+                        const float val = bilinear<S>(p, a.pitch, cs, row, nvf);
+                        float w = s.w;
+                        if constexpr (KIND == GCT_CYLINDRICAL) {
+                            const float nvk = fmaf(static_cast<float>(k), s.dnv, s.nv0);
+                            w = s.w / fmaf(nvk, nvk, s.rho2);
+                        }
+                        acc[k] = fmaf(val, w, acc[k]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!active) return;
+    const float sc = a.scale;
+#pragma unroll
+    for (int k = 0; k < ZU; ++k)
+        if (static_cast<uint32_t>(k) < zn) vp[k * slice] = a.finish ? acc[k] * sc : acc[k];
+}
+
+// 2-D images: each block of views has its (padded) detector rows staged whole
+// in LDS; a 16 x 16 pixel tile per workgroup.
+template <int KIND, int S>
+__global__ __launch_bounds__(256) void bp2d_lds_kernel(BpLaunch a, uint32_t vb)
+{
+    using T = typename Stor<S>::T;
+    extern __shared__ float rows[];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nbx = (a.nx + 15u) / 16u;
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t ix = (t % nbx) * 16u + (tid & 15u), iy = (t / nbx) * 16u + (tid >> 4);
+    const bool active = ix < a.nx && iy < a.ny;
+    float *vp = a.vol + static_cast<size_t>(active ? iy : 0u) * a.nx + (active ? ix : 0u);
+    float acc = (active && a.accumulate) ? *vp : 0.0f;
+    const float fx = static_cast<float>(ix), fy = static_cast<float>(iy);
+    const float nuf = static_cast<float>(a.nu);
+    const size_t view_elems = static_cast<size_t>(a.rows_pv) * a.pitch;
+    const T *base = static_cast<const T *>(a.filt);
+    const uint32_t vend = a.first_view + a.num_views;
+    for (uint32_t vb0 = a.first_view; vb0 < vend; vb0 += vb) {
+        const uint32_t nvb = min(vb, vend - vb0);
+        for (uint32_t j = 0; j < nvb; ++j) {
+            const T *p = base + static_cast<size_t>(vb0 + j) * view_elems + a.pitch;
+            for (uint32_t c = tid; c < a.pitch; c += 256u) rows[j * a.pitch + c] = Stor<S>::ld(p, c);
+        }
+        __syncthreads();
+        if (active) {
+            for (uint32_t j = 0; j < nvb; ++j) {
+                const BpRec &r = a.rec[vb0 + j];
+                float col, w;
+                if constexpr (KIND == GCT_PARALLEL_1D) {
+                    col = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
+                    w = 1.0f;
+                } else if constexpr (KIND == GCT_FAN_LINE_1D) {
+                    const float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
+                    const float nu = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
+                    const float rd = 1.0f / nw;
+                    col = fmaf(nu, rd, r.k[0]);
+                    w = rd * rd;
+                } else {
+                    const float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
+                    const float nu = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
+                    col = fmaf(atan2f(nu, nw), a.inv_dgamma, r.k[0]);
+                    w = r.k[3] / fmaf(nu, nu, nw * nw);
+                }
+                const Sample cs = col_sample(col, nuf);
+                const float *rr = rows + j * a.pitch + cs.colbase;
+                const float t0 = rr[0], t1 = rr[1];
+                acc = fmaf(fmaf(cs.alpha, t1 - t0, t0), w, acc);
+            }
+        }
+        __syncthreads();
+    }
+    if (active) *vp = a.finish ? acc * a.scale : acc;
 }
 
 // ---- persistent batched backprojection -------------------------------------
@@ -789,11 +1000,25 @@ void launch_bp3d_zu(const BpLaunch &a, hipStream_t s)
     }
 }
 
+template <int KIND, int S, bool F64>
+void launch_bp3d_lds(const BpLaunch &a, hipStream_t s)
+{
+    const uint32_t nbx = (a.nx + 15u) / 16u, nby = (a.ny + 15u) / 16u;
+    const uint32_t zu = (a.z_unroll == 32u) ? 32u : 16u;
+    const uint32_t blocks = nbx * nby * ((a.nz + zu - 1u) / zu);
+    if (zu == 32u) hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 32, 80, 2>), dim3(blocks), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 48, 4>), dim3(blocks), dim3(256), 0, s, a);
+}
+
 template <int KIND, int S>
 void launch_bp3d_variant(const BpLaunch &a, hipStream_t s)
 {
+    const bool lds = a.variant == GCT_BP_LDS;
     if constexpr (KIND == GCT_FLAT || KIND == GCT_CYLINDRICAL) {
-        if (a.setup_f64) {
+        if (lds && a.circular) {
+            if (a.setup_f64) launch_bp3d_lds<KIND, S, true>(a, s);
+            else launch_bp3d_lds<KIND, S, false>(a, s);
+        } else if (a.setup_f64) {
             if (a.circular) launch_bp3d_zu<KIND, S, true, true>(a, s);
             else launch_bp3d_zu<KIND, S, false, true>(a, s);
         } else {
@@ -801,7 +1026,21 @@ void launch_bp3d_variant(const BpLaunch &a, hipStream_t s)
             else launch_bp3d_zu<KIND, S, false, false>(a, s);
         }
     } else {
-        launch_bp3d_zu<KIND, S, true, false>(a, s);
+        if (lds) launch_bp3d_lds<KIND, S, false>(a, s);
+        else launch_bp3d_zu<KIND, S, true, false>(a, s);
+    }
+}
+
+template <int KIND, int S>
+void launch_bp2d(const BpLaunch &a, hipStream_t s)
+{
+    if (a.variant == GCT_BP_LDS) {
+        const uint32_t vb = std::max(1u, std::min(16u, 32768u / (a.pitch * 4u)));
+        const uint32_t blocks = ((a.nx + 15u) / 16u) * ((a.ny + 15u) / 16u);
+        hipLaunchKernelGGL((bp2d_lds_kernel<KIND, S>), dim3(blocks), dim3(256), vb * a.pitch * sizeof(float), s, a, vb);
+    } else {
+        const uint32_t blocks = ((a.nx + 7u) / 8u) * ((a.ny + 7u) / 8u);
+        hipLaunchKernelGGL((bp2d_kernel<KIND, S>), dim3(blocks), dim3(64), 0, s, a);
     }
 }
 
@@ -809,11 +1048,10 @@ template <int S>
 void launch_bp_S(const BpLaunch &a, hipStream_t s)
 {
     if (is_1d(a.kind)) {
-        const uint32_t blocks = ((a.nx + 7u) / 8u) * ((a.ny + 7u) / 8u);
         switch (a.kind) {
-        case GCT_PARALLEL_1D: hipLaunchKernelGGL((bp2d_kernel<GCT_PARALLEL_1D, S>), dim3(blocks), dim3(64), 0, s, a); break;
-        case GCT_FAN_ARC_1D: hipLaunchKernelGGL((bp2d_kernel<GCT_FAN_ARC_1D, S>), dim3(blocks), dim3(64), 0, s, a); break;
-        default: hipLaunchKernelGGL((bp2d_kernel<GCT_FAN_LINE_1D, S>), dim3(blocks), dim3(64), 0, s, a); break;
+        case GCT_PARALLEL_1D: launch_bp2d<GCT_PARALLEL_1D, S>(a, s); break;
+        case GCT_FAN_ARC_1D: launch_bp2d<GCT_FAN_ARC_1D, S>(a, s); break;
+        default: launch_bp2d<GCT_FAN_LINE_1D, S>(a, s); break;
         }
         return;
     }

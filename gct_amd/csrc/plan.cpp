@@ -99,6 +99,7 @@ size_t storage_bytes(int storage) { return storage == GCT_STORE_F32 ? 4u : 2u; }
 
 struct Tuning {
     uint32_t z_unroll;
+    int variant;
 };
 std::mutex g_tune_mu;
 std::map<std::tuple<int, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, int, int, int>, Tuning> g_tune;
@@ -125,7 +126,8 @@ struct gct_plan {
     bool setup_f64 = false;
     float inv_dgamma = 0.0f;
     float scale = 1.0f;
-    uint32_t z_unroll = 8;
+    uint32_t z_unroll = 16;
+    int variant = GCT_BP_LDS;
     DevBuf d_brec, d_brec64, d_frec, d_red, d_H, d_tw, d_filt, d_views;
 };
 
@@ -262,8 +264,11 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
     validate_image(geom, img);
     if (geom.det_row.num_samples > 2048u) fail(GCT_ERR_UNSUPPORTED, "more than 2048 detector columns");
     if (o.storage < GCT_STORE_F32 || o.storage > GCT_STORE_BF16) fail(GCT_ERR_INVALID_ARGUMENT, "unknown storage");
-    if (o.z_unroll != 0 && o.z_unroll != 4 && o.z_unroll != 8 && o.z_unroll != 16)
-        fail(GCT_ERR_INVALID_ARGUMENT, "z_unroll must be 0, 4, 8 or 16");
+    if (o.bp_kernel < GCT_BP_AUTO || o.bp_kernel > GCT_BP_LDS) fail(GCT_ERR_INVALID_ARGUMENT, "unknown bp_kernel");
+    const int variant = (o.bp_kernel == GCT_BP_DIRECT) ? GCT_BP_DIRECT : GCT_BP_LDS;
+    if (o.z_unroll != 0 && !(variant == GCT_BP_DIRECT ? (o.z_unroll == 4 || o.z_unroll == 8 || o.z_unroll == 16)
+                                                      : (o.z_unroll == 16 || o.z_unroll == 32)))
+        fail(GCT_ERR_INVALID_ARGUMENT, "z_unroll must be 4, 8 or 16 (direct kernel) or 16 or 32 (LDS kernel)");
     auto p = std::make_unique<gct_plan>();
     p->device = o.device;
     p->opts = o;
@@ -352,12 +357,16 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
     p->d_filt = DevBuf(fbytes);
     hip_check(hipMemset(p->d_filt.p, 0, fbytes), "hipMemset");   // zero padding stays zero
 
-    p->z_unroll = o.z_unroll ? o.z_unroll : 8u;
-    if (!o.z_unroll) {
+    p->variant = variant;
+    p->z_unroll = o.z_unroll ? o.z_unroll : (variant == GCT_BP_LDS ? 16u : 8u);
+    if (!o.z_unroll && o.bp_kernel == GCT_BP_AUTO) {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find({geom.kind, img.x.num_samples, img.y.num_samples, img.dim == 3 ? img.z.num_samples : 1u,
                                p->V, p->nu, p->nv, o.storage, int(p->circular), int(p->setup_f64)});
-        if (it != g_tune.end()) p->z_unroll = it->second.z_unroll;
+        if (it != g_tune.end()) {
+            p->z_unroll = it->second.z_unroll;
+            p->variant = it->second.variant;
+        }
     }
     return p;
 }
@@ -418,6 +427,7 @@ BpLaunch bp_args(gct_plan &p, float *vol, uint32_t first, uint32_t n, bool accum
     a.accumulate = accumulate;
     a.finish = finish;
     a.z_unroll = p.z_unroll;
+    a.variant = p.variant;
     return a;
 }
 
@@ -534,6 +544,7 @@ extern "C" gct_status gct_plan_get_info(const gct_plan *p, gct_plan_info *info)
         info->setup_f64 = p->setup_f64;
         info->circular = p->circular;
         info->storage = p->opts.storage;
+        info->bp_kernel = p->variant;
     });
 }
 
@@ -723,12 +734,15 @@ extern "C" gct_status gct_plan_autotune(gct_plan *p, float *vol, uint32_t *best_
         hipEvent_t e0, e1;
         hip_check(hipEventCreate(&e0), "event");
         hip_check(hipEventCreate(&e1), "event");
-        const uint32_t cands[3] = {4u, 8u, 16u};
+        struct Cand { int variant; uint32_t zu; };
+        const Cand cands[5] = {{GCT_BP_LDS, 16u}, {GCT_BP_LDS, 32u}, {GCT_BP_DIRECT, 4u}, {GCT_BP_DIRECT, 8u},
+                               {GCT_BP_DIRECT, 16u}};
         float best = 1e30f;
         uint32_t bz = p->z_unroll;
-        const uint32_t saved = p->z_unroll;
-        for (uint32_t zu : cands) {
-            p->z_unroll = zu;
+        int bv = p->variant;
+        for (const Cand &c : cands) {
+            p->z_unroll = c.zu;
+            p->variant = c.variant;
             const BpLaunch a = bp_args(*p, vol, 0, p->V, false, true, 0);
             launch_backproject(a, s);   // warm-up
             float tot = 0.0f;
@@ -742,17 +756,17 @@ extern "C" gct_status gct_plan_autotune(gct_plan *p, float *vol, uint32_t *best_
                 hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
                 tot += ms;
             }
-            if (tot / reps < best) { best = tot / reps; bz = zu; }
+            if (tot / reps < best) { best = tot / reps; bz = c.zu; bv = c.variant; }
         }
+        p->variant = bv;
         p->z_unroll = bz;
-        (void)saved;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         (void)hipStreamDestroy(s);
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             g_tune[{p->geom.kind, p->img.x.num_samples, p->img.y.num_samples, p->img.dim == 3 ? p->img.z.num_samples : 1u,
-                    p->V, p->nu, p->nv, p->opts.storage, int(p->circular), int(p->setup_f64)}] = Tuning{bz};
+                    p->V, p->nu, p->nv, p->opts.storage, int(p->circular), int(p->setup_f64)}] = Tuning{bz, bv};
         }
         if (best_zu) *best_zu = bz;
         if (best_ms) *best_ms = best;
@@ -831,6 +845,7 @@ void batch_prepare(gct_batch &b, const float *const *proj, float *const *vol)
         BatchScan sc{};
         sc.bp = bp_args(p, vol[oi], 0, p.V, false, true, 0);
         sc.bp.z_unroll = 8;
+        sc.bp.variant = GCT_BP_DIRECT;
         sc.tiles_x = (p.img.x.num_samples + 15u) / 16u;
         sc.tiles_y = (p.img.y.num_samples + 15u) / 16u;
         sc.tiles_z = is_1d(p.geom.kind) ? 1u : (p.img.z.num_samples + 7u) / 8u;

@@ -1,0 +1,460 @@
+// gct.hpp -- the C++ drop-in API of the MI355X FDK library.
+//
+// Same type names, fields and call shape as the reference's public API
+// (satidev/gct: src/cpp/geom, src/cpp/data, src/cuda/recon/recon_wrappers.cuh),
+// header-only over the C ABI in gct_amd.h.  A caller of
+//     GCT::Recon::CUDA::reconstruct(AxialCTGeom<...>, ProjectionStack, ReconImg&)
+// calls GCT::Recon::reconstruct with the same arguments; errors surface as the
+// same exception types (std::invalid_argument / std::runtime_error).
+#pragma once
+
+#include "gct_amd.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <numbers>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+namespace GCT {
+
+namespace Detail {
+inline void check(gct_status s)
+{
+    if (s == GCT_OK) return;
+    const std::string msg = gct_last_error();
+    if (s == GCT_ERR_INVALID_ARGUMENT) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+}  // namespace Detail
+
+// ---- units (src/cpp/geom/units.h) --------------------------------------------
+namespace Geom::Units {
+struct MilliMeters { float val; };
+struct DetPixels { float val; };
+struct Radians { float val; };
+constexpr Radians operator""_rad(long double v) { return Radians{static_cast<float>(v)}; }
+constexpr Radians operator""_deg(long double v)
+{
+    return Radians{static_cast<float>(std::numbers::pi_v<long double> * v / 180.0)};
+}
+constexpr MilliMeters operator""_mm(long double v) { return MilliMeters{static_cast<float>(v)}; }
+constexpr DetPixels operator""_pix(long double v) { return DetPixels{static_cast<float>(v)}; }
+template <class U> constexpr bool operator==(U a, U b) requires requires { a.val; } { return a.val == b.val; }
+template <class U> constexpr bool operator<(U a, U b) requires requires { a.val; } { return a.val < b.val; }
+template <class U> constexpr bool operator>(U a, U b) requires requires { a.val; } { return a.val > b.val; }
+}  // namespace Geom::Units
+
+// ---- geometry (src/cpp/geom) -------------------------------------------------
+namespace Geom {
+
+template <size_t Dim> struct VecWCS;
+template <> struct VecWCS<2u> { Units::MilliMeters x, y; };
+template <> struct VecWCS<3u> { Units::MilliMeters x, y, z; };
+
+template <typename WidthUnit> struct Sampler1D {
+    unsigned num_samples;
+    WidthUnit sampling_width;
+};
+using EqDistSampler = Sampler1D<Units::MilliMeters>;
+using EqAngSampler = Sampler1D<Units::Radians>;
+
+struct DetParallel { EqDistSampler sampler; };
+struct DetFanArc { EqAngSampler sampler; };
+struct DetFanLine { EqDistSampler sampler; };
+
+template <typename DetRow, typename... DetCol> struct ProjGeom;
+template <> struct ProjGeom<DetParallel> {
+    Units::Radians rot_ang;
+    Units::DetPixels det_row_center;
+};
+template <> struct ProjGeom<DetFanArc> {
+    Units::MilliMeters src_det_dist;
+    Units::DetPixels det_row_center;
+    VecWCS<2u> src_pos, det_row_dir, iso_ray_dir;
+};
+template <> struct ProjGeom<DetFanLine> {
+    Units::MilliMeters src_det_dist;
+    Units::DetPixels det_row_center;
+    VecWCS<2u> src_pos, det_row_dir;
+};
+template <> struct ProjGeom<DetParallel, DetParallel> {
+    Units::Radians rot_ang;
+    Units::DetPixels det_row_center, det_col_center;
+};
+template <> struct ProjGeom<DetParallel, DetFanLine> {
+    Units::MilliMeters src_det_dist, src_iso_dist;
+    Units::Radians rot_ang;
+    Units::DetPixels det_row_center, det_col_center;
+};
+template <> struct ProjGeom<DetFanArc, DetFanLine> {
+    Units::MilliMeters src_det_dist;
+    Units::DetPixels det_row_center, det_col_center;
+    VecWCS<3u> src_pos, det_row_dir, det_col_dir, iso_ray_dir;
+};
+template <> struct ProjGeom<DetFanLine, DetFanLine> {
+    Units::MilliMeters src_det_dist;
+    Units::DetPixels det_row_center, det_col_center;
+    VecWCS<3u> src_pos, det_row_dir, det_col_dir;
+};
+
+enum class RotDir { CCW = -1, CW = 1 };
+
+struct RotationGeom {
+    RotDir dir;
+    Units::Radians rot_ang_incr;
+    Units::Radians rot_ang_range;
+    unsigned num_proj_per_rot;
+    std::vector<Units::Radians> rot_ang;
+    std::vector<Units::Radians> rel_rot_ang;
+};
+
+template <typename DetRow, typename... DetCol> constexpr int32_t geomKind()
+{
+    if constexpr (sizeof...(DetCol) == 0) {
+        if constexpr (std::is_same_v<DetRow, DetParallel>) return GCT_PARALLEL_1D;
+        else if constexpr (std::is_same_v<DetRow, DetFanArc>) return GCT_FAN_ARC_1D;
+        else return GCT_FAN_LINE_1D;
+    } else {
+        using DC = std::tuple_element_t<0, std::tuple<DetCol...>>;
+        if constexpr (std::is_same_v<DetRow, DetParallel> && std::is_same_v<DC, DetParallel>) return GCT_PARALLEL_2D;
+        else if constexpr (std::is_same_v<DetRow, DetParallel>) return GCT_CONE_PARALLEL;
+        else if constexpr (std::is_same_v<DetRow, DetFanArc>) return GCT_CYLINDRICAL;
+        else return GCT_FLAT;
+    }
+}
+
+namespace Detail {
+inline void put(float *d, VecWCS<2u> const &v) { d[0] = v.x.val; d[1] = v.y.val; d[2] = 0.0f; }
+inline void put(float *d, VecWCS<3u> const &v) { d[0] = v.x.val; d[1] = v.y.val; d[2] = v.z.val; }
+inline void get(float const *d, VecWCS<2u> &v) { v.x.val = d[0]; v.y.val = d[1]; }
+inline void get(float const *d, VecWCS<3u> &v) { v.x.val = d[0]; v.y.val = d[1]; v.z.val = d[2]; }
+
+template <typename DetRow, typename... DetCol>
+gct_proj_geom toC(ProjGeom<DetRow, DetCol...> const &pg)
+{
+    gct_proj_geom g{};
+    using P = ProjGeom<DetRow, DetCol...>;
+    if constexpr (requires(P p) { p.rot_ang; }) g.rot_ang = pg.rot_ang.val;
+    if constexpr (requires(P p) { p.src_det_dist; }) g.src_det_dist = pg.src_det_dist.val;
+    if constexpr (requires(P p) { p.src_iso_dist; }) g.src_iso_dist = pg.src_iso_dist.val;
+    g.det_row_center = pg.det_row_center.val;
+    if constexpr (requires(P p) { p.det_col_center; }) g.det_col_center = pg.det_col_center.val;
+    if constexpr (requires(P p) { p.src_pos; }) put(g.src_pos, pg.src_pos);
+    if constexpr (requires(P p) { p.det_row_dir; }) put(g.det_row_dir, pg.det_row_dir);
+    if constexpr (requires(P p) { p.det_col_dir; }) put(g.det_col_dir, pg.det_col_dir);
+    if constexpr (requires(P p) { p.iso_ray_dir; }) put(g.iso_ray_dir, pg.iso_ray_dir);
+    return g;
+}
+
+template <typename DetRow, typename... DetCol>
+ProjGeom<DetRow, DetCol...> fromC(gct_proj_geom const &g)
+{
+    ProjGeom<DetRow, DetCol...> pg{};
+    using P = ProjGeom<DetRow, DetCol...>;
+    if constexpr (requires(P p) { p.rot_ang; }) pg.rot_ang.val = g.rot_ang;
+    if constexpr (requires(P p) { p.src_det_dist; }) pg.src_det_dist.val = g.src_det_dist;
+    if constexpr (requires(P p) { p.src_iso_dist; }) pg.src_iso_dist.val = g.src_iso_dist;
+    pg.det_row_center.val = g.det_row_center;
+    if constexpr (requires(P p) { p.det_col_center; }) pg.det_col_center.val = g.det_col_center;
+    if constexpr (requires(P p) { p.src_pos; }) get(g.src_pos, pg.src_pos);
+    if constexpr (requires(P p) { p.det_row_dir; }) get(g.det_row_dir, pg.det_row_dir);
+    if constexpr (requires(P p) { p.det_col_dir; }) get(g.det_col_dir, pg.det_col_dir);
+    if constexpr (requires(P p) { p.iso_ray_dir; }) get(g.iso_ray_dir, pg.iso_ray_dir);
+    return pg;
+}
+
+template <typename Det> gct_sampler sampler(Det const &d)
+{
+    return gct_sampler{d.sampler.num_samples, d.sampler.sampling_width.val};
+}
+}  // namespace Detail
+
+// CTGeom / AxialCTGeom (src/cpp/geom/ct_geom.h:16-53, axial_ct_geom.h:8-13):
+// the constructor analyses the trajectory and throws for an invalid one.
+template <typename DetRow, typename... DetCol>
+struct CTGeom {
+public:
+    CTGeom(std::tuple<DetRow, DetCol...> const &t_dg, std::vector<ProjGeom<DetRow, DetCol...>> const &t_fpg)
+        : dg{t_dg}, fpg{t_fpg}
+    {
+        cviews.reserve(fpg.size());
+        for (auto const &p : fpg) cviews.push_back(Detail::toC(p));
+        gct_ct_geom g = c();
+        gct_rotation_geom r{};
+        std::vector<float> rot(fpg.size()), rel(fpg.size());
+        if (gct_rotation_geom_compute(&g, &r, rot.data(), rel.data()) != GCT_OK)
+            throw std::runtime_error{"Invalid X-ray source trajectory.\n"};
+        rg.dir = r.dir < 0 ? RotDir::CCW : RotDir::CW;
+        rg.rot_ang_incr = Units::Radians{r.rot_ang_incr};
+        rg.rot_ang_range = Units::Radians{r.rot_ang_range};
+        rg.num_proj_per_rot = r.num_proj_per_rot;
+        for (size_t i = 0; i < fpg.size(); ++i) {
+            rg.rot_ang.push_back(Units::Radians{rot[i]});
+            rg.rel_rot_ang.push_back(Units::Radians{rel[i]});
+        }
+    }
+    std::tuple<DetRow, DetCol...> const &detGeom() const noexcept { return dg; }
+    std::vector<ProjGeom<DetRow, DetCol...>> const &projGeom() const noexcept { return fpg; }
+    RotationGeom const &rotationGeom() const noexcept { return rg; }
+
+    // The C-ABI view of this geometry (valid while *this lives).
+    gct_ct_geom c() const
+    {
+        gct_ct_geom g{};
+        g.kind = geomKind<DetRow, DetCol...>();
+        g.det_row = Detail::sampler(std::get<0>(dg));
+        if constexpr (sizeof...(DetCol) > 0) g.det_col = Detail::sampler(std::get<1>(dg));
+        else g.det_col = gct_sampler{1u, 1.0f};
+        g.num_proj = static_cast<uint32_t>(cviews.size());
+        g.proj_geom = cviews.data();
+        return g;
+    }
+
+protected:
+    std::tuple<DetRow, DetCol...> dg;
+    std::vector<ProjGeom<DetRow, DetCol...>> fpg;
+    RotationGeom rg;
+    std::vector<gct_proj_geom> cviews;
+};
+
+template <typename DetRow, typename... DetCol>
+struct AxialCTGeom : public CTGeom<DetRow, DetCol...> {
+    using CTGeom<DetRow, DetCol...>::CTGeom;
+};
+
+// ReconImgGeom<2|3> (src/cpp/geom/recon_img_geom.h:12-30).
+template <size_t Dim> struct ReconImgGeom;
+template <> struct ReconImgGeom<2u> {
+    EqDistSampler x, y;
+    VecWCS<2u> mid_point;
+};
+template <> struct ReconImgGeom<3u> {
+    EqDistSampler x, y, z;
+    VecWCS<3u> mid_point;
+};
+
+// Acquisition parameters (src/cpp/io/parameters/proj_geom_params.h:16-59).
+enum class WCS { DCM };
+struct RotationParams {
+    RotDir dir = RotDir::CW;
+    Units::Radians init_ang;
+    Units::Radians ang_incr;
+    unsigned num_proj;
+    float hel_pitch = 0.0f;
+};
+struct Parallel1DProjGeomParams {
+    Units::DetPixels det_row_center;
+    WCS wcs = WCS::DCM;
+    RotationParams rot;
+};
+struct FanBeamProjGeomParams {
+    Units::DetPixels det_row_center;
+    WCS wcs = WCS::DCM;
+    RotationParams rot;
+    Units::MilliMeters src_det_dist;
+    Units::MilliMeters src_iso_dist;
+};
+struct Parallel2DProjGeomParams {
+    Units::DetPixels det_row_center;
+    WCS wcs = WCS::DCM;
+    RotationParams rot;
+    Units::DetPixels det_col_center;
+};
+struct ConeBeamProjGeomParams {
+    Units::DetPixels det_row_center;
+    WCS wcs = WCS::DCM;
+    RotationParams rot;
+    Units::MilliMeters src_det_dist;
+    Units::MilliMeters src_iso_dist;
+    Units::DetPixels det_col_center;
+};
+
+// ProjGeomFactory<...>::create (src/cpp/io/geom_reader/proj_geom_factory_params.cpp:8-108).
+template <typename DetRow, typename... DetCol>
+struct ProjGeomFactory {
+    template <typename Params>
+    static ProjGeom<DetRow, DetCol...> create(Params const &p, unsigned proj_idx)
+    {
+        gct_circular_params c{};
+        c.kind = geomKind<DetRow, DetCol...>();
+        c.dir = static_cast<int32_t>(p.rot.dir);
+        c.init_ang = p.rot.init_ang.val;
+        c.ang_incr = p.rot.ang_incr.val;
+        c.num_proj = proj_idx + 1u;
+        c.hel_pitch = p.rot.hel_pitch;
+        c.det_row_center = p.det_row_center.val;
+        if constexpr (requires(Params q) { q.det_col_center; }) c.det_col_center = p.det_col_center.val;
+        if constexpr (requires(Params q) { q.src_det_dist; }) c.src_det_dist = p.src_det_dist.val;
+        if constexpr (requires(Params q) { q.src_iso_dist; }) c.src_iso_dist = p.src_iso_dist.val;
+        std::vector<gct_proj_geom> out(c.num_proj);
+        GCT::Detail::check(gct_geom_create_circular(&c, out.data()));
+        return Detail::fromC<DetRow, DetCol...>(out[proj_idx]);
+    }
+};
+
+}  // namespace Geom
+
+// ---- data containers (src/cpp/data) ---------------------------------------------
+namespace Data {
+
+template <size_t ProjDim> struct ProjSize;
+template <> struct ProjSize<1u> { unsigned num_cols; };
+template <> struct ProjSize<2u> { unsigned num_cols, num_rows; };
+
+template <typename... DetCol> constexpr size_t projDim() { return sizeof...(DetCol) + 1u; }
+template <typename... DetCol> constexpr size_t reconImgDim() { return sizeof...(DetCol) + 2u; }
+
+inline unsigned numProjPixels(ProjSize<1u> const &s) { return s.num_cols; }
+inline unsigned numProjPixels(ProjSize<2u> const &s) { return s.num_cols * s.num_rows; }
+inline unsigned numProjRows(ProjSize<1u> const &) { return 1u; }
+inline unsigned numProjRows(ProjSize<2u> const &s) { return s.num_rows; }
+
+// x (detector column) fastest, then row, then view (projection_stack.h:14).
+template <typename PixType, size_t ProjDim>
+class ProjectionStack {
+public:
+    ProjectionStack(ProjSize<ProjDim> const &size, unsigned num_proj, PixType init_val = PixType{})
+        : proj_size_{size}, num_proj_{num_proj}
+    {
+        if (numProjPixels(size) == 0 || num_proj == 0)
+            throw std::invalid_argument{"Projections size or number of projections cannot be zero.\n"};
+        elems_.assign(size_t(numProjPixels(size)) * num_proj, init_val);
+    }
+    ProjSize<ProjDim> projSize() const noexcept { return proj_size_; }
+    unsigned numProj() const noexcept { return num_proj_; }
+    std::vector<PixType> const &elements() const { return elems_; }
+    std::vector<PixType> &elements() { return elems_; }
+    size_t numElements() const noexcept { return elems_.size(); }
+    void setData(std::vector<PixType> data)
+    {
+        if (data.size() != elems_.size())
+            throw std::invalid_argument{"Cannot copy data to the projection stack: wrong number of elements.\n"};
+        elems_ = std::move(data);
+    }
+
+private:
+    ProjSize<ProjDim> proj_size_;
+    unsigned num_proj_;
+    std::vector<PixType> elems_;
+};
+
+// x fastest, then y, then z (recon_img.h, back_proj.cuh:76).
+template <typename PixType, size_t Dim>
+class ReconImg {
+public:
+    explicit ReconImg(Geom::ReconImgGeom<Dim> const &geom) : geom_{geom}
+    {
+        size_t n = size_t(geom.x.num_samples) * geom.y.num_samples;
+        if constexpr (Dim == 3u) n *= geom.z.num_samples;
+        bool ok = geom.x.sampling_width.val > 0.0f && geom.y.sampling_width.val > 0.0f;
+        if constexpr (Dim == 3u) ok = ok && geom.z.sampling_width.val > 0.0f;
+        if (n == 0) throw std::invalid_argument{"The number of samples in any dimension cannot be zero.\n"};
+        if (!ok) throw std::invalid_argument{"The sampling width of the image in any dimension cannot be zero or negative.\n"};
+        elems_.assign(n, PixType{});
+    }
+    Geom::ReconImgGeom<Dim> const &geom() const noexcept { return geom_; }
+    std::vector<PixType> const &elements() const { return elems_; }
+    size_t numElements() const noexcept { return elems_.size(); }
+    void setData(std::vector<PixType> &&data)
+    {
+        if (data.size() != elems_.size())
+            throw std::invalid_argument{"Cannot move data to the recon image: wrong number of elements.\n"};
+        elems_ = std::move(data);
+    }
+    gct_image_geom c() const
+    {
+        gct_image_geom g{};
+        g.dim = static_cast<uint32_t>(Dim);
+        g.x = gct_sampler{geom_.x.num_samples, geom_.x.sampling_width.val};
+        g.y = gct_sampler{geom_.y.num_samples, geom_.y.sampling_width.val};
+        g.mid_point[0] = geom_.mid_point.x.val;
+        g.mid_point[1] = geom_.mid_point.y.val;
+        if constexpr (Dim == 3u) {
+            g.z = gct_sampler{geom_.z.num_samples, geom_.z.sampling_width.val};
+            g.mid_point[2] = geom_.mid_point.z.val;
+        } else {
+            g.z = gct_sampler{1u, 1.0f};
+        }
+        return g;
+    }
+
+private:
+    Geom::ReconImgGeom<Dim> geom_;
+    std::vector<PixType> elems_;
+};
+
+}  // namespace Data
+
+// ---- reconstruction (src/cuda/recon/recon_wrappers.cuh:13-45) -----------------
+namespace Recon {
+
+enum class ApodFunc { NONE, HANN, HAMMING };
+struct FiltParams {
+    float cut_off_freq = 1.0f;
+    ApodFunc apod_func = ApodFunc::NONE;
+};
+
+// isValidAxialRecon (src/cpp/recon/check_validity.h:66-75).
+template <typename DetRow, typename... DetCol>
+bool isValidAxialRecon(Geom::CTGeom<DetRow, DetCol...> const &cg,
+                       Data::ProjectionStack<float, Data::projDim<DetCol...>()> const &proj, bool verbose = false)
+{
+    gct_ct_geom g = cg.c();
+    const auto ps = proj.projSize();
+    uint32_t rows = 1u;
+    if constexpr (sizeof...(DetCol) > 0) rows = ps.num_rows;
+    const gct_status s = gct_check_axial_recon(&g, ps.num_cols, rows, proj.numProj());
+    (void)verbose;
+    return s == GCT_OK;
+}
+
+namespace Detail {
+template <typename DetRow, typename... DetCol>
+void reconstructImpl(Geom::AxialCTGeom<DetRow, DetCol...> const &cg,
+                     Data::ProjectionStack<float, Data::projDim<DetCol...>()> const &proj,
+                     Data::ReconImg<float, Data::reconImgDim<DetCol...>()> &vol, gct_recon_opts const *opts)
+{
+    gct_ct_geom g = cg.c();
+    gct_image_geom im = vol.c();
+    std::vector<float> out(vol.numElements());
+    GCT::Detail::check(gct_reconstruct(&g, proj.elements().data(), &im, out.data(), opts));
+    vol.setData(std::move(out));   // the result replaces the image's data (fdk_axial.cuh:202-204)
+}
+}  // namespace Detail
+
+// The seven overloads of Recon::CUDA::reconstruct.
+inline void reconstruct(Geom::AxialCTGeom<Geom::DetParallel> const &cg, Data::ProjectionStack<float, 1u> const &proj,
+                        Data::ReconImg<float, 2u> &vol, gct_recon_opts const *opts = nullptr)
+{ Detail::reconstructImpl(cg, proj, vol, opts); }
+inline void reconstruct(Geom::AxialCTGeom<Geom::DetFanArc> const &cg, Data::ProjectionStack<float, 1u> const &proj,
+                        Data::ReconImg<float, 2u> &vol, gct_recon_opts const *opts = nullptr)
+{ Detail::reconstructImpl(cg, proj, vol, opts); }
+inline void reconstruct(Geom::AxialCTGeom<Geom::DetFanLine> const &cg, Data::ProjectionStack<float, 1u> const &proj,
+                        Data::ReconImg<float, 2u> &vol, gct_recon_opts const *opts = nullptr)
+{ Detail::reconstructImpl(cg, proj, vol, opts); }
+inline void reconstruct(Geom::AxialCTGeom<Geom::DetParallel, Geom::DetParallel> const &cg,
+                        Data::ProjectionStack<float, 2u> const &proj, Data::ReconImg<float, 3u> &vol,
+                        gct_recon_opts const *opts = nullptr)
+{ Detail::reconstructImpl(cg, proj, vol, opts); }
+inline void reconstruct(Geom::AxialCTGeom<Geom::DetParallel, Geom::DetFanLine> const &cg,
+                        Data::ProjectionStack<float, 2u> const &proj, Data::ReconImg<float, 3u> &vol,
+                        gct_recon_opts const *opts = nullptr)
+{ Detail::reconstructImpl(cg, proj, vol, opts); }
+inline void reconstruct(Geom::AxialCTGeom<Geom::DetFanArc, Geom::DetFanLine> const &cg,
+                        Data::ProjectionStack<float, 2u> const &proj, Data::ReconImg<float, 3u> &vol,
+                        gct_recon_opts const *opts = nullptr)
+{ Detail::reconstructImpl(cg, proj, vol, opts); }
+inline void reconstruct(Geom::AxialCTGeom<Geom::DetFanLine, Geom::DetFanLine> const &cg,
+                        Data::ProjectionStack<float, 2u> const &proj, Data::ReconImg<float, 3u> &vol,
+                        gct_recon_opts const *opts = nullptr)
+{ Detail::reconstructImpl(cg, proj, vol, opts); }
+
+}  // namespace Recon
+}  // namespace GCT

@@ -123,6 +123,13 @@ typedef enum gct_mem_mode {
     GCT_MEM_LOW = 2                  /* views streamed in chunks */
 } gct_mem_mode;
 
+/* Backprojection kernel family. */
+typedef enum gct_bp_kernel {
+    GCT_BP_AUTO = 0,                 /* tuned choice, else LDS-staged */
+    GCT_BP_DIRECT = 1,               /* taps read from HBM/L2 through the vector cache */
+    GCT_BP_LDS = 2                   /* per-tile detector footprint staged in LDS */
+} gct_bp_kernel;
+
 typedef struct gct_recon_opts {
     float cut_off_freq;              /* FiltParams::cut_off_freq, default 1 */
     int32_t window;                  /* gct_window, default NONE (the reference's FiltParams{}) */
@@ -132,6 +139,7 @@ typedef struct gct_recon_opts {
     int32_t device;                  /* HIP device ordinal, default 0 (fdk_axial.cuh:73) */
     uint32_t z_unroll;               /* BP voxels per thread along z; 0 = tuned */
     uint32_t view_chunk;             /* LOW mode views per chunk; 0 = derived */
+    int32_t bp_kernel;               /* gct_bp_kernel, default AUTO */
 } gct_recon_opts;
 
 /* ---- library ----------------------------------------------------------- */
@@ -223,6 +231,7 @@ typedef struct gct_plan_info {
     int32_t setup_f64;
     int32_t circular;                /* 1 when every view keeps the row axis along z */
     int32_t storage;
+    int32_t bp_kernel;               /* gct_bp_kernel in use */
 } gct_plan_info;
 
 /* Builds per-view tables (FP64 on the host, stored FP32), the redundancy

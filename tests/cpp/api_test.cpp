@@ -1,0 +1,160 @@
+// Exercises the header-only GCT:: drop-in API over the C ABI.
+//   api_test            geometry known answers (no GPU needed)
+//   api_test --gpu DIR  also reconstructs a flat-panel scan through
+//                       GCT::Recon::reconstruct and writes DIR/vol.raw
+// Known answers follow the reference's tests:
+//   test/cpp/geom/proj_geom_factory_params_test.cpp (views at 270 and 0 deg),
+//   test/cpp/geom/rotation_geom_test.cpp (rotation analysis),
+//   test/cpp/recon/check_validity_test.cpp (isValidAxialRecon).
+#include "gct/gct.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <numbers>
+#include <string>
+
+using namespace GCT;
+using namespace GCT::Geom;
+using namespace GCT::Geom::Units;
+
+static int failures = 0;
+#define CHECK(c)                                                             \
+    do {                                                                     \
+        if (!(c)) {                                                          \
+            std::printf("FAIL %s:%d %s\n", __FILE__, __LINE__, #c);          \
+            ++failures;                                                      \
+        }                                                                    \
+    } while (0)
+
+static bool near(float a, float b, float eps = 1e-4f) { return std::fabs(a - b) <= eps; }
+
+static RotationParams rot4()
+{
+    RotationParams r;
+    r.dir = RotDir::CW;
+    r.init_ang = Radians{std::numbers::pi_v<float> / 2.0f};
+    r.ang_incr = Radians{std::numbers::pi_v<float> / 2.0f};
+    r.num_proj = 4u;
+    return r;
+}
+
+int main(int argc, char **argv)
+{
+    // 1-D parallel: third view at 270 deg, fourth at 360 deg (exact float sums)
+    {
+        Parallel1DProjGeomParams p;
+        p.rot = rot4();
+        p.det_row_center = 256.0_pix;
+        auto pg2 = ProjGeomFactory<DetParallel>::create(p, 2u);
+        auto pg3 = ProjGeomFactory<DetParallel>::create(p, 3u);
+        CHECK(pg2.rot_ang.val == 3.0f * std::numbers::pi_v<float> / 2.0f);
+        CHECK(pg3.rot_ang.val == 2.0f * std::numbers::pi_v<float>);
+        CHECK(pg2.det_row_center.val == 256.0f);
+    }
+    // fan arc: at 270 deg the source is on -y, rows along +x, iso-ray along +y
+    {
+        FanBeamProjGeomParams p;
+        p.rot = rot4();
+        p.det_row_center = 256.0_pix;
+        p.src_iso_dist = 500.0_mm;
+        p.src_det_dist = 1000.0_mm;
+        auto pg = ProjGeomFactory<DetFanArc>::create(p, 2u);
+        CHECK(near(pg.src_pos.x.val, 0.0f) && near(pg.src_pos.y.val, -500.0f));
+        CHECK(near(pg.det_row_dir.x.val, 1.0f) && near(pg.det_row_dir.y.val, 0.0f));
+        CHECK(near(pg.iso_ray_dir.x.val, 0.0f) && near(pg.iso_ray_dir.y.val, 1.0f));
+        auto pg0 = ProjGeomFactory<DetFanArc>::create(p, 3u);
+        CHECK(near(pg0.src_pos.x.val, 500.0f) && near(pg0.src_pos.y.val, 0.0f));
+        CHECK(near(pg0.det_row_dir.x.val, 0.0f) && near(pg0.det_row_dir.y.val, 1.0f));
+        CHECK(near(pg0.iso_ray_dir.x.val, -1.0f) && near(pg0.iso_ray_dir.y.val, 0.0f));
+    }
+    // flat cone: at 270 deg, rows along +x, columns along -z
+    {
+        ConeBeamProjGeomParams p;
+        p.rot = rot4();
+        p.det_row_center = 256.0_pix;
+        p.det_col_center = 257.0_pix;
+        p.src_iso_dist = 500.0_mm;
+        p.src_det_dist = 1000.0_mm;
+        auto pg = ProjGeomFactory<DetFanLine, DetFanLine>::create(p, 2u);
+        CHECK(near(pg.src_pos.x.val, 0.0f) && near(pg.src_pos.y.val, -500.0f) && near(pg.src_pos.z.val, 0.0f));
+        CHECK(near(pg.det_row_dir.x.val, 1.0f) && near(pg.det_row_dir.y.val, 0.0f) && near(pg.det_row_dir.z.val, 0.0f));
+        CHECK(near(pg.det_col_dir.x.val, 0.0f) && near(pg.det_col_dir.y.val, 0.0f) && near(pg.det_col_dir.z.val, -1.0f));
+        CHECK(pg.det_col_center.val == 257.0f);
+    }
+    // rotation analysis of every geometry: CW, increment 90 deg, range 360 deg, 4 per rotation
+    {
+        ConeBeamProjGeomParams p;
+        p.rot = rot4();
+        p.det_row_center = 256.0_pix;
+        p.det_col_center = 257.0_pix;
+        p.src_iso_dist = 500.0_mm;
+        p.src_det_dist = 1000.0_mm;
+        std::vector<ProjGeom<DetFanLine, DetFanLine>> v;
+        for (unsigned i = 0; i < 4; ++i) v.push_back(ProjGeomFactory<DetFanLine, DetFanLine>::create(p, i));
+        AxialCTGeom<DetFanLine, DetFanLine> cg{std::make_tuple(DetFanLine{{512u, 1.0_mm}}, DetFanLine{{512u, 1.0_mm}}), v};
+        auto const &rg = cg.rotationGeom();
+        CHECK(rg.dir == RotDir::CW);
+        CHECK(near(rg.rot_ang_incr.val, std::numbers::pi_v<float> / 2.0f));
+        CHECK(near(rg.rot_ang_range.val, 2.0f * std::numbers::pi_v<float>));
+        CHECK(rg.num_proj_per_rot == 4u);
+        // isValidAxialRecon: matching and mismatching stacks
+        Data::ProjectionStack<float, 2u> ok{{512u, 512u}, 4u};
+        Data::ProjectionStack<float, 2u> bad_cols{{511u, 512u}, 4u};
+        Data::ProjectionStack<float, 2u> bad_views{{512u, 512u}, 3u};
+        CHECK(Recon::isValidAxialRecon(cg, ok));
+        CHECK(!Recon::isValidAxialRecon(cg, bad_cols));
+        CHECK(!Recon::isValidAxialRecon(cg, bad_views));
+    }
+    // a stationary trajectory throws from the constructor (ct_geom.h:26-32)
+    {
+        std::vector<ProjGeom<DetParallel>> v(3);
+        for (auto &x : v) x.rot_ang = Radians{0.3f};
+        bool threw = false;
+        try {
+            AxialCTGeom<DetParallel> cg{std::make_tuple(DetParallel{{16u, 1.0_mm}}), v};
+        } catch (std::runtime_error const &) {
+            threw = true;
+        }
+        CHECK(threw);
+    }
+    // zero-size image throws std::invalid_argument (recon_img.h:42-55)
+    {
+        bool threw = false;
+        try {
+            Data::ReconImg<float, 2u> im{ReconImgGeom<2u>{{0u, 1.0_mm}, {4u, 1.0_mm}, {}}};
+        } catch (std::invalid_argument const &) {
+            threw = true;
+        }
+        CHECK(threw);
+    }
+
+    if (argc >= 3 && std::strcmp(argv[1], "--gpu") == 0) {
+        // flat-panel full scan through the drop-in call, 48^3 from a uniform sphere's exact projections
+        ConeBeamProjGeomParams p;
+        p.rot.init_ang = Radians{0.0f};
+        p.rot.ang_incr = Radians{2.0f * std::numbers::pi_v<float> / 90.0f};
+        p.rot.num_proj = 90u;
+        p.det_row_center = 39.5_pix;
+        p.det_col_center = 31.5_pix;
+        p.src_iso_dist = 200.0_mm;
+        p.src_det_dist = 400.0_mm;
+        std::vector<ProjGeom<DetFanLine, DetFanLine>> v;
+        for (unsigned i = 0; i < 90u; ++i) v.push_back(ProjGeomFactory<DetFanLine, DetFanLine>::create(p, i));
+        AxialCTGeom<DetFanLine, DetFanLine> cg{std::make_tuple(DetFanLine{{80u, 1.0_mm}}, DetFanLine{{64u, 1.0_mm}}), v};
+        Data::ProjectionStack<float, 2u> proj{{80u, 64u}, 90u};
+        std::ifstream in(std::string(argv[2]) + "/proj.raw", std::ios::binary);
+        std::vector<float> buf(proj.numElements());
+        in.read(reinterpret_cast<char *>(buf.data()), static_cast<std::streamsize>(buf.size() * sizeof(float)));
+        proj.setData(buf);
+        CHECK(Recon::isValidAxialRecon(cg, proj));
+        Data::ReconImg<float, 3u> vol{ReconImgGeom<3u>{{48u, 0.5_mm}, {48u, 0.5_mm}, {48u, 0.5_mm}, {}}};
+        Recon::reconstruct(cg, proj, vol);
+        std::ofstream out(std::string(argv[2]) + "/vol.raw", std::ios::binary);
+        out.write(reinterpret_cast<const char *>(vol.elements().data()),
+                  static_cast<std::streamsize>(vol.elements().size() * sizeof(float)));
+    }
+    std::printf("%s (%d failures)\n", failures ? "FAILED" : "OK", failures);
+    return failures ? 1 : 0;
+}
