@@ -405,3 +405,58 @@ def phantom_rasterize(img: Image, params: np.ndarray) -> np.ndarray:
 
 def device_count() -> int:
     return int(lib().gct_device_count())
+
+
+class Server:
+    """Continuous-batching reconstruction server on one device (gct_server)."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, max_inflight: int = 0, num_streams: int = 0):
+        o = ServerOpts(device, hbm_budget_bytes, max_inflight, num_streams)
+        h = C.c_void_p()
+        check(lib().gct_server_create(C.byref(o), C.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def submit(self, scan: Scan, img: Image, proj: np.ndarray, out: np.ndarray, reproject: bool = False, **kw) -> int:
+        """Queues one request; proj and out must stay alive until wait() returns."""
+        proj = np.ascontiguousarray(proj, np.float32)
+        assert out.dtype == np.float32 and out.flags.c_contiguous and out.size == img.nvox
+        o = opts(**kw)
+        t = C.c_uint64()
+        check(lib().gct_server_submit(self.h, C.byref(scan.c()), C.byref(img.c()), C.byref(o), proj.ctypes.data,
+                                      out.ctypes.data, int(reproject), C.byref(t)))
+        self._keep[t.value] = (proj, out, scan)
+        return t.value
+
+    def wait(self, ticket: int):
+        """(latency_ms, reprojection error or -1); raises GctError if the request failed."""
+        lat, err = C.c_double(), C.c_double()
+        st = lib().gct_server_wait(self.h, ticket, C.byref(lat), C.byref(err))
+        self._keep.pop(ticket, None)
+        check(st)
+        return lat.value, err.value
+
+    def poll(self, ticket: int):
+        lat, err = C.c_double(), C.c_double()
+        st = lib().gct_server_poll(self.h, ticket, C.byref(lat), C.byref(err))
+        if st == 7:
+            return None
+        self._keep.pop(ticket, None)
+        check(st)
+        return lat.value, err.value
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        check(lib().gct_server_get_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().gct_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

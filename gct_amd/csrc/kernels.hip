@@ -953,6 +953,35 @@ __global__ __launch_bounds__(256) void joseph_kernel(FpLaunch a)
     a.proj[idx] = joseph_ray(a, o, d, from_src);
 }
 
+// ---- residual of a reprojection -------------------------------------------------
+
+// partials[2 b] = sum over block b of (a - b)^2, partials[2 b + 1] = sum of b^2.
+__global__ __launch_bounds__(256) void residual_kernel(const float *__restrict__ a, const float *__restrict__ b,
+                                                       size_t n, double *partials)
+{
+    __shared__ double s1[256], s2[256];
+    double x = 0.0, y = 0.0;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * 256u + threadIdx.x; i < n; i += static_cast<size_t>(gridDim.x) * 256u) {
+        const double d = static_cast<double>(a[i]) - static_cast<double>(b[i]);
+        x += d * d;
+        y += static_cast<double>(b[i]) * static_cast<double>(b[i]);
+    }
+    s1[threadIdx.x] = x;
+    s2[threadIdx.x] = y;
+    __syncthreads();
+    for (uint32_t w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+            s2[threadIdx.x] += s2[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[2u * blockIdx.x] = s1[0];
+        partials[2u * blockIdx.x + 1u] = s2[0];
+    }
+}
+
 // ---- dispatch helpers -------------------------------------------------------
 
 template <int KIND, int S>
@@ -1134,6 +1163,12 @@ void launch_unpack_filtered(const void *filt, float *dst, int storage, uint32_t 
     default: hipLaunchKernelGGL(unpack_kernel<GCT_STORE_F32>, dim3(blocks), dim3(256), 0, s, filt, dst, nu, nv, nviews, pitch, rows_pv); break;
     }
     hip_check(hipGetLastError(), "unpack filtered projections");
+}
+
+void launch_residual(const float *a, const float *b, size_t n, double *partials, uint32_t blocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(residual_kernel, dim3(blocks), dim3(256), 0, s, a, b, n, partials);
+    hip_check(hipGetLastError(), "residual");
 }
 
 void launch_backproject_batch(const BatchScan *scans_dev, uint32_t num_scans, uint32_t total_tiles, uint32_t *counter,

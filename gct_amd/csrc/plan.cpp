@@ -7,7 +7,7 @@
 // (src/cuda/recon/fdk_axial.cuh:53-205), CTGeomDev (src/cuda/geom/ct_geom_dev.cu),
 // RampFilteringFFT (src/cuda/recon/ramp_filtering_fft.cuh:79-111),
 // getParkerWeights (src/cuda/recon/parker_weighting.cuh:132-200).
-#include "common.hpp"
+#include "plan_internal.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -22,6 +22,8 @@
 
 namespace gct {
 
+size_t storage_bytes(int storage) { return storage == GCT_STORE_F32 ? 4u : 2u; }
+
 namespace {
 thread_local std::string g_last_error;
 }
@@ -31,46 +33,6 @@ void set_last_error(const std::string &msg) { g_last_error = msg; }
 namespace {
 
 // ---- device memory --------------------------------------------------------
-
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    explicit DevBuf(size_t n) : bytes(n)
-    {
-        if (n) hip_check(hipMalloc(&p, n), "hipMalloc");
-    }
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
-    DevBuf &operator=(DevBuf &&o) noexcept
-    {
-        if (this != &o) {
-            release();
-            p = o.p; bytes = o.bytes;
-            o.p = nullptr; o.bytes = 0;
-        }
-        return *this;
-    }
-    ~DevBuf() { release(); }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    template <class T> T *as() const { return static_cast<T *>(p); }
-};
-
-struct DeviceGuard {
-    int prev = 0;
-    explicit DeviceGuard(int dev)
-    {
-        hip_check(hipGetDevice(&prev), "hipGetDevice");
-        if (dev != prev) hip_check(hipSetDevice(dev), "hipSetDevice");
-    }
-    ~DeviceGuard() { (void)hipSetDevice(prev); }
-};
 
 // ---- host FFT (FP64) for the filter spectrum ------------------------------
 
@@ -95,8 +57,6 @@ void fft64(std::vector<std::complex<double>> &a)
     }
 }
 
-size_t storage_bytes(int storage) { return storage == GCT_STORE_F32 ? 4u : 2u; }
-
 struct Tuning {
     uint32_t z_unroll;
     int variant;
@@ -109,27 +69,6 @@ std::map<std::tuple<int, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint3
 }  // namespace gct
 
 using namespace gct;
-
-// ---- the plan ---------------------------------------------------------------
-
-struct gct_plan {
-    int device = 0;
-    gct_ct_geom geom{};
-    std::vector<gct_proj_geom> views;
-    gct_image_geom img{};
-    gct_recon_opts opts{};
-    RotationInfo rot;
-    uint32_t nu = 0, nv = 0, V = 0, L = 0;
-    uint32_t pitch = 0, rows_pv = 0;
-    uint32_t filt_views = 0;   // views the filtered buffer holds
-    bool circular = true;
-    bool setup_f64 = false;
-    float inv_dgamma = 0.0f;
-    float scale = 1.0f;
-    uint32_t z_unroll = 16;
-    int variant = GCT_BP_LDS;
-    DevBuf d_brec, d_brec64, d_frec, d_red, d_H, d_tw, d_filt, d_views;
-};
 
 namespace gct {
 namespace {
@@ -251,14 +190,19 @@ void check_basis(const gct_plan &p)
     }
 }
 
-void upload(DevBuf &b, const void *src, size_t bytes)
+void upload(gct_plan &p, DevBuf &b, const void *src, size_t bytes)
 {
-    b = DevBuf(bytes);
-    hip_check(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+    b = DevBuf(bytes, p.stream, p.ordered);
+    hip_check(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, p.stream), "hipMemcpyAsync H2D");
 }
 
+// Builds a plan.  All device work (uploads, the redundancy table, clearing the
+// filtered buffer) runs on `stream` and is complete when this returns; with
+// ordered = true the buffers come from the stream-ordered pool.
+}  // namespace
+
 std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geom &img, const gct_recon_opts &o,
-                                    uint32_t filt_views)
+                                    uint32_t filt_views, hipStream_t stream, bool ordered)
 {
     validate_geom(geom);
     validate_image(geom, img);
@@ -271,6 +215,8 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
         fail(GCT_ERR_INVALID_ARGUMENT, "z_unroll must be 4, 8 or 16 (direct kernel) or 16 or 32 (LDS kernel)");
     auto p = std::make_unique<gct_plan>();
     p->device = o.device;
+    p->stream = stream;
+    p->ordered = ordered;
     p->opts = o;
     p->views.assign(geom.proj_geom, geom.proj_geom + geom.num_proj);
     p->geom = geom;
@@ -297,10 +243,10 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
     std::vector<BpRec64> rec64;
     std::vector<FiltRec> frec;
     build_records(*p, rec, rec64, frec);
-    upload(p->d_brec, rec.data(), rec.size() * sizeof(BpRec));
-    if (p->setup_f64) upload(p->d_brec64, rec64.data(), rec64.size() * sizeof(BpRec64));
-    upload(p->d_frec, frec.data(), frec.size() * sizeof(FiltRec));
-    upload(p->d_views, p->views.data(), p->views.size() * sizeof(gct_proj_geom));
+    upload(*p, p->d_brec, rec.data(), rec.size() * sizeof(BpRec));
+    if (p->setup_f64) upload(*p, p->d_brec64, rec64.data(), rec64.size() * sizeof(BpRec64));
+    upload(*p, p->d_frec, frec.data(), frec.size() * sizeof(FiltRec));
+    upload(*p, p->d_views, p->views.data(), p->views.size() * sizeof(gct_proj_geom));
 
     // Filter spectrum: FFT_L of the N-tap kernel placed at n mod L, scaled by
     // 1/L (the cuFFT C2R normalisation of ramp_filtering_fft.cu:59), from the
@@ -320,13 +266,13 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
         const double a = -2.0 * std::numbers::pi * static_cast<double>(k) / static_cast<double>(p->L);
         tw[k] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
     }
-    upload(p->d_H, H.data(), H.size() * sizeof(float2));
-    upload(p->d_tw, tw.data(), tw.size() * sizeof(float2));
+    upload(*p, p->d_H, H.data(), H.size() * sizeof(float2));
+    upload(*p, p->d_tw, tw.data(), tw.size() * sizeof(float2));
 
     // Redundancy table [V][Nu] (getParkerWeights, parker_weighting.cuh:132-200):
     // fan angles from view 0 on the host in FP32 as the reference does
     // (fanAngles :61-77), the table on the device.
-    p->d_red = DevBuf(size_t(p->V) * p->nu * sizeof(float));
+    p->d_red = DevBuf(size_t(p->V) * p->nu * sizeof(float), p->stream, p->ordered);
     {
         int mode = 0;
         float value = 1.0f;
@@ -346,16 +292,16 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
         // range <= pi for a fan geometry: the reference leaves the table
         // uninitialised (parker_weighting.cuh:163-175); this library uses 1.
         DevBuf dfan, dbeta;
-        upload(dfan, fan.data(), fan.size() * sizeof(float));
-        upload(dbeta, beta.data(), beta.size() * sizeof(float));
+        upload(*p, dfan, fan.data(), fan.size() * sizeof(float));
+        upload(*p, dbeta, beta.data(), beta.size() * sizeof(float));
         launch_redundancy(geom.kind, dfan.as<float>(), dbeta.as<float>(), p->nu, p->V, p->rot.range, mode, value,
-                          p->d_red.as<float>(), nullptr);
-        hip_check(hipDeviceSynchronize(), "redundancy weights");
+                          p->d_red.as<float>(), p->stream);
+        const size_t fbytes = size_t(p->filt_views) * p->rows_pv * p->pitch * storage_bytes(o.storage);
+        p->d_filt = DevBuf(fbytes, p->stream, p->ordered);
+        hip_check(hipMemsetAsync(p->d_filt.p, 0, fbytes, p->stream), "hipMemsetAsync");   // zero padding stays zero
+        // host vectors above must outlive their copies; dfan/dbeta free in order
+        hip_check(hipStreamSynchronize(p->stream), "plan set-up");
     }
-
-    const size_t fbytes = size_t(p->filt_views) * p->rows_pv * p->pitch * storage_bytes(o.storage);
-    p->d_filt = DevBuf(fbytes);
-    hip_check(hipMemset(p->d_filt.p, 0, fbytes), "hipMemset");   // zero padding stays zero
 
     p->variant = variant;
     p->z_unroll = o.z_unroll ? o.z_unroll : (variant == GCT_BP_LDS ? 16u : 8u);
@@ -431,10 +377,38 @@ BpLaunch bp_args(gct_plan &p, float *vol, uint32_t first, uint32_t n, bool accum
     return a;
 }
 
+namespace {
+
 void check_range(const gct_plan &p, uint32_t first, uint32_t n)
 {
     if (first > p.V || n > p.V - first) fail(GCT_ERR_INVALID_ARGUMENT, "view range outside the scan");
     if (n > p.filt_views) fail(GCT_ERR_INVALID_ARGUMENT, "view range larger than the plan's filtered buffer");
+}
+
+}  // namespace
+
+FpLaunch fp_args(gct_plan &p, const float *vol, float *proj)
+{
+    FpLaunch a{};
+    a.kind = p.geom.kind;
+    a.nx = p.img.x.num_samples;
+    a.ny = p.img.y.num_samples;
+    a.nz = (p.img.dim == 3) ? p.img.z.num_samples : 1u;
+    a.dx = p.img.x.sampling_width;
+    a.dy = p.img.y.sampling_width;
+    a.dz = (p.img.dim == 3) ? p.img.z.sampling_width : 1.0f;
+    a.x0 = static_cast<float>(double(p.img.mid_point[0]) - (double(a.nx) - 1.0) * a.dx / 2.0);
+    a.y0 = static_cast<float>(double(p.img.mid_point[1]) - (double(a.ny) - 1.0) * a.dy / 2.0);
+    a.z0 = (p.img.dim == 3) ? static_cast<float>(double(p.img.mid_point[2]) - (double(a.nz) - 1.0) * a.dz / 2.0) : 0.0f;
+    a.nu = p.nu;
+    a.nv = p.nv;
+    a.nviews = p.V;
+    a.du = p.geom.det_row.sampling_width;
+    a.dv = is_1d(p.geom.kind) ? 1.0f : p.geom.det_col.sampling_width;
+    a.views = p.d_views.as<gct_proj_geom>();
+    a.vol = vol;
+    a.proj = proj;
+    return a;
 }
 
 uint64_t plan_device_bytes(const gct_ct_geom &g, const gct_image_geom &img, const gct_recon_opts &o, uint32_t fviews)
@@ -454,7 +428,6 @@ uint64_t volume_bytes(const gct_image_geom &img)
     return uint64_t(img.x.num_samples) * img.y.num_samples * (img.dim == 3 ? img.z.num_samples : 1u) * sizeof(float);
 }
 
-}  // namespace
 }  // namespace gct
 
 // ---- C ABI ------------------------------------------------------------------
@@ -497,7 +470,7 @@ extern "C" int gct_device_count(void)
     return n;
 }
 
-static gct_recon_opts opts_or_default(const gct_recon_opts *o)
+gct_recon_opts gct::opts_or_default(const gct_recon_opts *o)
 {
     gct_recon_opts d;
     gct_recon_opts_default(&d);
@@ -607,25 +580,7 @@ extern "C" gct_status gct_plan_forward_project(gct_plan *p, const float *vol, fl
     return guarded([&] {
         if (!p || !vol || !proj) fail(GCT_ERR_INVALID_ARGUMENT, "null argument");
         DeviceGuard dg(p->device);
-        FpLaunch a{};
-        a.kind = p->geom.kind;
-        a.nx = p->img.x.num_samples;
-        a.ny = p->img.y.num_samples;
-        a.nz = (p->img.dim == 3) ? p->img.z.num_samples : 1u;
-        a.dx = p->img.x.sampling_width;
-        a.dy = p->img.y.sampling_width;
-        a.dz = (p->img.dim == 3) ? p->img.z.sampling_width : 1.0f;
-        a.x0 = static_cast<float>(double(p->img.mid_point[0]) - (double(a.nx) - 1.0) * a.dx / 2.0);
-        a.y0 = static_cast<float>(double(p->img.mid_point[1]) - (double(a.ny) - 1.0) * a.dy / 2.0);
-        a.z0 = (p->img.dim == 3) ? static_cast<float>(double(p->img.mid_point[2]) - (double(a.nz) - 1.0) * a.dz / 2.0) : 0.0f;
-        a.nu = p->nu;
-        a.nv = p->nv;
-        a.nviews = p->V;
-        a.du = p->geom.det_row.sampling_width;
-        a.dv = is_1d(p->geom.kind) ? 1.0f : p->geom.det_col.sampling_width;
-        a.views = p->d_views.as<gct_proj_geom>();
-        a.vol = vol;
-        a.proj = proj;
+        const FpLaunch a = fp_args(*p, vol, proj);
         launch_forward_project(a, static_cast<hipStream_t>(stream));
     });
 }

@@ -1,0 +1,110 @@
+// Internals shared by the host sources of libgct_amd: device buffers, the
+// plan object and the launch-argument builders.  Not part of the C ABI.
+#pragma once
+
+#include "common.hpp"
+
+#include <memory>
+
+namespace gct {
+
+// Device allocation; with a stream it is stream-ordered (hipMallocAsync /
+// hipFreeAsync from the device's memory pool), so a serving request neither
+// allocates nor frees with a device-wide synchronisation.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    hipStream_t stream = nullptr;
+    bool ordered = false;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) : bytes(n)
+    {
+        if (n) hip_check(hipMalloc(&p, n), "hipMalloc");
+    }
+    DevBuf(size_t n, hipStream_t s, bool stream_ordered) : bytes(n), stream(s), ordered(stream_ordered)
+    {
+        if (!n) return;
+        if (ordered) hip_check(hipMallocAsync(&p, n, s), "hipMallocAsync");
+        else hip_check(hipMalloc(&p, n), "hipMalloc");
+    }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes), stream(o.stream), ordered(o.ordered)
+    {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf &operator=(DevBuf &&o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p; bytes = o.bytes; stream = o.stream; ordered = o.ordered;
+            o.p = nullptr; o.bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) {
+            if (ordered) (void)hipFreeAsync(p, stream);
+            else (void)hipFree(p);
+        }
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+struct DeviceGuard {
+    int prev = 0;
+    explicit DeviceGuard(int dev)
+    {
+        hip_check(hipGetDevice(&prev), "hipGetDevice");
+        if (dev != prev) hip_check(hipSetDevice(dev), "hipSetDevice");
+    }
+    ~DeviceGuard() { (void)hipSetDevice(prev); }
+};
+
+}  // namespace gct
+
+// ---- the plan ---------------------------------------------------------------
+
+struct gct_plan {
+    using DevBuf = gct::DevBuf;
+    int device = 0;
+    gct_ct_geom geom{};
+    std::vector<gct_proj_geom> views;
+    gct_image_geom img{};
+    gct_recon_opts opts{};
+    gct::RotationInfo rot;
+    uint32_t nu = 0, nv = 0, V = 0, L = 0;
+    uint32_t pitch = 0, rows_pv = 0;
+    uint32_t filt_views = 0;   // views the filtered buffer holds
+    bool circular = true;
+    bool setup_f64 = false;
+    float inv_dgamma = 0.0f;
+    float scale = 1.0f;
+    uint32_t z_unroll = 16;
+    int variant = GCT_BP_LDS;
+    hipStream_t stream = nullptr;     // stream-ordered plans (serving) allocate and free on it
+    bool ordered = false;
+    DevBuf d_brec, d_brec64, d_frec, d_red, d_H, d_tw, d_filt, d_views;
+};
+
+
+namespace gct {
+
+std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geom &img, const gct_recon_opts &o,
+                                    uint32_t filt_views, hipStream_t stream = nullptr, bool ordered = false);
+FilterLaunch filter_args(gct_plan &p, const float *proj, uint32_t proj_view_offset, uint32_t first, uint32_t n,
+                         uint32_t filt_view_offset);
+BpLaunch bp_args(gct_plan &p, float *vol, uint32_t first, uint32_t n, bool accumulate, bool finish,
+                 uint32_t filt_view_offset);
+FpLaunch fp_args(gct_plan &p, const float *vol, float *proj);
+uint64_t plan_device_bytes(const gct_ct_geom &g, const gct_image_geom &img, const gct_recon_opts &o, uint32_t fviews);
+uint64_t volume_bytes(const gct_image_geom &img);
+size_t storage_bytes(int storage);
+gct_recon_opts opts_or_default(const gct_recon_opts *o);
+
+}  // namespace gct

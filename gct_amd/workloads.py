@@ -122,7 +122,7 @@ def small_case(kind: int, n: int = 64, views: int = 120, nu: int = 96, nv: int =
         scan = circular_scan(kind, nu, 1.0, views, incr, det_row_center=(nu - 1) / 2.0 + 0.25, src_det_dist=sdd,
                              src_iso_dist=sid, direction=direction)
     elif kind == FAN_ARC_1D:
-        scan = circular_scan(kind, nu, 0.5 / nu * 1.6, views, incr, det_row_center=(nu - 1) / 2.0 - 0.3,
+        scan = circular_scan(kind, nu, 0.5 / sid, views, incr, det_row_center=(nu - 1) / 2.0 - 0.3,
                              src_det_dist=sdd, src_iso_dist=sid, direction=direction)
     elif kind == PARALLEL_2D:
         scan = circular_scan(kind, nu, 0.6, views, incr, nv=nv, dv=0.6, det_row_center=(nu - 1) / 2.0,
@@ -131,7 +131,7 @@ def small_case(kind: int, n: int = 64, views: int = 120, nu: int = 96, nv: int =
         scan = circular_scan(kind, nu, 0.6, views, incr, nv=nv, dv=0.9, det_row_center=(nu - 1) / 2.0,
                              det_col_center=(nv - 1) / 2.0, src_det_dist=sdd, src_iso_dist=sid, direction=direction)
     elif kind == CYLINDRICAL:
-        scan = circular_scan(kind, nu, 0.5 / nu * 1.6, views, incr, nv=nv, dv=1.0, det_row_center=(nu - 1) / 2.0,
+        scan = circular_scan(kind, nu, 0.5 / sid, views, incr, nv=nv, dv=1.0, det_row_center=(nu - 1) / 2.0,
                              det_col_center=(nv - 1) / 2.0, src_det_dist=sdd, src_iso_dist=sid, direction=direction)
     else:
         scan = circular_scan(kind, nu, 1.0, views, incr, nv=nv, dv=1.0, det_row_center=(nu - 1) / 2.0 + 0.5,

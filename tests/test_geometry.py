@@ -1,0 +1,243 @@
+"""Host side of the library (no GPU): geometry factories, rotation analysis,
+ramp kernel, projection matrix, validity checks, phantoms, ABI surface."""
+import ctypes as C
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import gct_amd as g
+import oracle_lib as O
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+# ---- ABI surface ---------------------------------------------------------------------
+
+def test_library_exports_every_declared_symbol():
+    header = (ROOT / "include" / "gct_amd.h").read_text()
+    names = set(re.findall(r"\b(gct_[a-z0-9_]+)\s*\(", header))
+    assert len(names) > 30
+    L = g.lib()
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+    assert L.gct_abi_version() == 1
+
+
+def test_no_device_reports_status_without_crashing():
+    if g.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    scan = g.circular_scan(g.FLAT, 16, 1.0, 8, math.pi / 4, nv=8, dv=1.0, src_det_dist=400, src_iso_dist=200)
+    with pytest.raises(g.GctError) as e:
+        g.Plan(scan, g.Image(8, 8, 1.0, 1.0, 8, 1.0))
+    assert e.value.status == 4   # GCT_ERR_NO_DEVICE
+
+
+# ---- factories (proj_geom_factory_params_test.cpp) ------------------------------------
+
+def _four_views(kind):
+    # init 90 deg, increment 90 deg, 4 views, CW; SID 500, SDD 1000; centres 256/257
+    return g.circular_scan(kind, 512, 1.0, 4, np.float32(math.pi / 2), init_ang=np.float32(math.pi / 2), nv=512,
+                           dv=1.0, det_row_center=256.0, det_col_center=257.0, src_det_dist=1000.0,
+                           src_iso_dist=500.0)
+
+
+@pytest.mark.parametrize("kind", [g.PARALLEL_1D, g.PARALLEL_2D, g.CONE_PARALLEL])
+def test_parallel_factory_rotation_angles(kind):
+    v = _four_views(kind).views
+    assert v[2, 0] == np.float32(3.0) * np.float32(math.pi) / np.float32(2.0)
+    assert v[3, 0] == np.float32(2.0) * np.float32(math.pi)
+    assert v[2, 3] == 256.0
+
+
+@pytest.mark.parametrize("kind", [g.FAN_ARC_1D, g.FAN_LINE_1D, g.CYLINDRICAL, g.FLAT])
+def test_fan_cone_factory_known_answers(kind):
+    s = _four_views(kind)
+    pg = s.views
+    src, eu, ev, ew = pg[:, 5:8], pg[:, 8:11], pg[:, 11:14], pg[:, 14:17]
+    # third view (270 deg): source on -y, rows along +x, iso-ray along +y
+    assert np.allclose(src[2, :2], [0.0, -500.0], atol=1e-4)
+    assert np.allclose(eu[2, :2], [1.0, 0.0], atol=1e-4)
+    assert np.allclose(ew[2, :2], [0.0, 1.0], atol=1e-4)
+    # fourth view (0 deg): source on +x, rows along +y, iso-ray along -x
+    assert np.allclose(src[3, :2], [500.0, 0.0], atol=1e-4)
+    assert np.allclose(eu[3, :2], [0.0, 1.0], atol=1e-4)
+    assert np.allclose(ew[3, :2], [-1.0, 0.0], atol=1e-4)
+    assert np.allclose(np.hypot(src[:, 0], src[:, 1]), 500.0, rtol=1e-6)
+    if not g.is_1d(kind):
+        assert np.allclose(ev[:, 2], -1.0, atol=1e-6) and np.allclose(ev[:, :2], 0.0)
+
+
+def test_factory_matches_oracle_restatement_of_eigen_rotation():
+    """src = Rz(dir theta) (SID, 0, 0); basis rows of implicit * Rz^T, in FP32."""
+    s = g.circular_scan(g.FLAT, 8, 1.0, 37, np.float32(0.17), init_ang=np.float32(0.3), nv=4, dv=1.0,
+                        src_det_dist=900.0, src_iso_dist=600.0, direction=-1)
+    for i in range(37):
+        th = np.float32(np.float32(0.3) + np.float32(0.17) * np.float32(i))
+        phi = np.float32(-1.0) * th
+        c, sn = np.float32(math.cos(float(phi))), np.float32(math.sin(float(phi)))
+        assert s.views[i, 5] == pytest.approx(float(c * np.float32(600.0)), abs=1e-3)
+        assert s.views[i, 6] == pytest.approx(float(sn * np.float32(600.0)), abs=1e-3)
+        assert s.views[i, 8] == pytest.approx(float(-sn), abs=1e-6)
+        assert s.views[i, 9] == pytest.approx(float(c), abs=1e-6)
+
+
+# ---- rotation analysis: product vs oracle (and reference build) -------------------------
+
+@pytest.mark.parametrize("kind", range(7), ids=g.KIND_NAMES)
+@pytest.mark.parametrize("span,direction,start", [(2 * math.pi, 1, 0.0), (math.radians(200), -1, 1.0),
+                                                  (math.pi, 1, 5.5), (4.0, -1, 3.0)])
+def test_rotation_geometry_matches_oracle(kind, span, direction, start):
+    s = g.circular_scan(kind, 64, 0.5, 123, np.float32(span / 123), init_ang=start, nv=8, dv=1.0,
+                        src_det_dist=800.0, src_iso_dist=400.0, direction=direction)
+    r, rot, rel = g.rotation_geom(s)
+    rc, ro, orot, orel = O.rotation(s)
+    assert rc == 0
+    assert (r.dir, r.rot_ang_incr, r.rot_ang_range, r.num_proj_per_rot) == (ro.dir, ro.incr, ro.range, ro.nppr)
+    assert np.array_equal(rot, orot) and np.array_equal(rel, orel)
+
+
+def test_invalid_trajectory_is_a_runtime_error():
+    v = np.zeros((5, g.PROJ_GEOM_FLOATS), np.float32)
+    v[:, 0] = 0.7
+    with pytest.raises(g.GctError) as e:
+        g.rotation_geom(g.Scan(g.PARALLEL_1D, 8, 1.0, views=v))
+    assert e.value.status == 2 and "trajectory" in str(e.value)
+
+
+# ---- ramp kernel: product vs oracle and reference ------------------------------------------
+
+@pytest.mark.parametrize("kind", [g.PARALLEL_1D, g.FAN_ARC_1D, g.FAN_LINE_1D, g.CYLINDRICAL, g.FLAT])
+@pytest.mark.parametrize("window", [0, 1, 2, 3])
+def test_ramp_kernel_matches_oracle(kind, window):
+    s = g.circular_scan(kind, 301, 0.0021 if kind in (g.FAN_ARC_1D, g.CYLINDRICAL) else 0.37, 4, 0.5, nv=4, dv=1.0,
+                        src_det_dist=800.0, src_iso_dist=400.0)
+    taps = g.ramp_kernel(s, 1.0, window)
+    t32, _ = O.ramp_taps(kind in (g.FAN_ARC_1D, g.CYLINDRICAL), 301, s.du, 1.0, window)
+    assert np.array_equal(taps, t32)
+    if O.ref_available() and window < 3:
+        ref = np.zeros(301, np.float32)
+        det = 2 if kind in (g.FAN_ARC_1D, g.CYLINDRICAL) else (0 if kind == g.PARALLEL_1D else 1)
+        O.ref().ref_ramp_taps(det, 301, s.du, 1.0, window, ref.ctypes.data)
+        assert np.array_equal(taps, ref)
+
+
+def test_shepp_logan_needs_unit_cutoff():
+    s = g.circular_scan(g.PARALLEL_1D, 16, 1.0, 4, 0.5)
+    with pytest.raises(g.GctError):
+        g.ramp_kernel(s, 0.8, g.WINDOW_SHEPP_LOGAN)
+
+
+# ---- projection matrix (projection_matrix_test.cpp, proj_geom_factory_proj_mat_test.cpp) --
+
+ARTIS = np.array([[2.48159483828274, 0.39171974522293, 0.0, 307.5],
+                  [0.0, 0.305095541401274, -2.48159483828274, 239.5],
+                  [0.0, 0.00127388535031847, 0.0, 1.0]], np.float64)
+
+
+def test_projection_matrix_decomposition_known_answer():
+    pg = g.ProjGeom()
+    focal = (C.c_double * 2)()
+    g.check(g.lib().gct_projection_matrix_decompose(ARTIS.ctypes.data, 0.616, C.byref(pg), focal))
+    assert pg.det_row_center == pytest.approx(307.5, rel=1e-6)
+    assert pg.det_col_center == pytest.approx(239.5, rel=1e-6)
+    assert focal[0] == pytest.approx(1200.0 / 0.616, rel=1e-6)
+    assert pg.src_det_dist == pytest.approx(1200.0, rel=1e-6)
+    assert list(pg.src_pos) == pytest.approx([0.0, -785.0, 0.0], abs=1e-3)
+    assert list(pg.det_row_dir) == pytest.approx([1.0, 0.0, 0.0], abs=1e-6)
+    assert list(pg.det_col_dir) == pytest.approx([0.0, 0.0, -1.0], abs=1e-6)
+    assert list(pg.iso_ray_dir) == pytest.approx([0.0, 1.0, 0.0], abs=1e-6)
+
+
+def test_projection_matrix_compose_decompose_round_trip():
+    s = g.circular_scan(g.FLAT, 616, 0.616, 248, np.float32(math.radians(200) / 248), nv=480, dv=0.616,
+                        det_row_center=307.5, det_col_center=239.5, src_det_dist=1200.0, src_iso_dist=750.0)
+    for i in (0, 61, 200):
+        P = np.zeros(12)
+        pg_in = g.ProjGeom.from_buffer_copy(s.views[i].tobytes())
+        g.check(g.lib().gct_projection_matrix(C.byref(pg_in), 0.616, 0.616, P.ctypes.data))
+        assert P[11] == pytest.approx(1.0)
+        out = g.ProjGeom()
+        g.check(g.lib().gct_projection_matrix_decompose(P.ctypes.data, 0.616, C.byref(out), None))
+        assert out.src_det_dist == pytest.approx(1200.0, rel=1e-5)
+        assert out.det_row_center == pytest.approx(307.5, abs=1e-3)
+        assert list(out.src_pos) == pytest.approx(list(s.views[i, 5:8]), abs=1e-2)
+        assert list(out.det_row_dir) == pytest.approx(list(s.views[i, 8:11]), abs=1e-5)
+        # a voxel projects to the same detector position through P and through the flat mapping
+        X = np.array([3.0, -7.0, 11.0, 1.0])
+        col, row = (P[0:4] @ X) / (P[8:12] @ X), (P[4:8] @ X) / (P[8:12] @ X)
+        src, eu, ev = s.views[i, 5:8].astype(np.float64), s.views[i, 8:11].astype(np.float64), s.views[i, 11:14].astype(np.float64)
+        ew = np.cross(eu, ev)
+        r = X[:3] - src
+        assert col == pytest.approx(1200.0 * (r @ eu) / (r @ ew) / 0.616 + 307.5, abs=1e-6)
+        assert row == pytest.approx(1200.0 * (r @ ev) / (r @ ew) / 0.616 + 239.5, abs=1e-6)
+
+
+# ---- validity checks (check_validity_test.cpp) -------------------------------------------
+
+def test_check_axial_recon():
+    s = g.circular_scan(g.FLAT, 32, 1.0, 10, 0.3, nv=16, dv=1.0, src_det_dist=400, src_iso_dist=200)
+    L = g.lib()
+    assert L.gct_check_axial_recon(C.byref(s.c()), 32, 16, 10) == 0
+    assert L.gct_check_axial_recon(C.byref(s.c()), 31, 16, 10) == 1
+    assert L.gct_check_axial_recon(C.byref(s.c()), 32, 15, 10) == 1
+    assert L.gct_check_axial_recon(C.byref(s.c()), 32, 16, 11) == 1
+    assert b"not available for all projection images" in L.gct_last_error()
+
+
+def test_required_memory_counts_projections_and_volume():
+    s = g.circular_scan(g.FLAT, 512, 0.4, 512, 2 * math.pi / 512, nv=384, dv=0.4, src_det_dist=1000, src_iso_dist=500)
+    img = g.Image(256, 256, 0.25, 0.25, 256, 0.25)
+    b = C.c_uint64()
+    g.check(g.lib().gct_required_memory(C.byref(s.c()), C.byref(img.c()), None, C.byref(b)))
+    assert b.value > 512 * 384 * 512 * 4 + 256 ** 3 * 4
+
+
+# ---- phantoms --------------------------------------------------------------------------------
+
+def test_phantom_sphere_chord_closed_form():
+    """Parallel rays through a ball of radius r: 2 sqrt(r^2 - t^2) times density."""
+    r, dens = 10.0, 0.25
+    ph = np.array([[0.0, 0.0, 0.0, r, r, r, 0.3, 1.1, 2.0, dens]])
+    s = g.circular_scan(g.PARALLEL_2D, 41, 0.5, 3, 0.7, nv=5, dv=0.5)
+    proj = g.phantom_project(s, ph)
+    t = (np.arange(41) - 20.0) * 0.5
+    z = (np.arange(5) - 2.0) * 0.5
+    want = 2.0 * np.sqrt(np.maximum(r * r - t[None, :] ** 2 - z[:, None] ** 2, 0.0)) * dens
+    for v in range(3):
+        assert np.allclose(proj[v], want, atol=1e-5)
+
+
+def test_phantom_ellipse_rotation_and_fan_rays():
+    """A rotated ellipse seen along its axes: chord 2a or 2b through the centre."""
+    a, b = 9.0, 4.0
+    ph = np.array([[0.0, 0.0, 0.0, a, b, 1.0, math.pi / 2, 0.0, 0.0, 1.0]])   # major axis along y
+    views = np.zeros((2, g.PROJ_GEOM_FLOATS), np.float32)
+    views[:, 0] = [0.0, np.float32(math.pi / 2)]
+    views[:, 3] = 2.0
+    s = g.Scan(g.PARALLEL_1D, 5, 1.0, views=views)
+    proj = g.phantom_project(s, ph)
+    # view 0 rays run along x (across the minor axis), view 1 along y (the major axis)
+    assert proj[0, 0, 2] == pytest.approx(2 * b, rel=1e-6)
+    assert proj[1, 0, 2] == pytest.approx(2 * a, rel=1e-6)
+
+
+def test_phantom_poisson_noise_statistics():
+    ph = np.array([[0.0, 0.0, 0.0, 50.0, 50.0, 50.0, 0, 0, 0, 0.02]])
+    s = g.circular_scan(g.PARALLEL_2D, 64, 0.1, 4, 0.7, nv=64, dv=0.1)
+    clean = g.phantom_project(s, ph)
+    noisy = g.phantom_project(s, ph, i0=1e5, noise_seed=3)
+    counts = 1e5 * np.exp(-noisy)
+    lam = 1e5 * np.exp(-clean)
+    z = (counts - lam) / np.sqrt(lam)
+    assert abs(z.mean()) < 0.05 and abs(z.std() - 1.0) < 0.05
+
+
+def test_phantom_random_is_seeded_and_inside_fov():
+    a = g.phantom_random(5, 12, 3, 32.0)
+    b = g.phantom_random(5, 12, 3, 32.0)
+    assert np.array_equal(a, b)
+    assert np.all(np.linalg.norm(a[:, :3], axis=1) <= 0.8 * 32.0 + 1e-9)
+    assert np.all((a[:, 3:6] >= 0.05 * 32) & (a[:, 3:6] <= 0.35 * 32))

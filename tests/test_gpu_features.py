@@ -1,0 +1,224 @@
+"""GPU tests of the features around the FDK path: precision forms, batched
+persistent launches in HIP graphs, the autotuner, serving, the Joseph
+projector.  All through the C ABI."""
+import math
+
+import numpy as np
+import pytest
+
+import gct_amd as g
+import oracle_lib as O
+from gct_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(torch, wl, proj, **kw):
+    plan = g.Plan(wl.scan, wl.img, **kw)
+    p = torch.from_numpy(proj).cuda()
+    v = torch.empty(wl.img.shape, dtype=torch.float32, device="cuda")
+    plan.execute(p, v)
+    torch.cuda.synchronize()
+    return v.cpu().numpy(), plan
+
+
+# ---- precision forms ------------------------------------------------------------------
+
+@pytest.mark.parametrize("storage,bound", [(g.STORE_F16, 1e-3), (g.STORE_BF16, 1e-3)], ids=["f16", "bf16"])
+@pytest.mark.parametrize("kind", [g.FAN_LINE_1D, g.CYLINDRICAL, g.FLAT], ids=lambda k: g.KIND_NAMES[k])
+def test_half_storage_rmse_against_fp64_oracle(torch_dev, kind, storage, bound):
+    """Filtered projections stored in FP16/BF16, FP32 accumulation: RMSE against
+    the FP64 oracle within 1e-3 (values are O(0.1) mm^-1)."""
+    wl = W.small_case(kind)
+    proj = wl.projections()
+    got, plan = _run(torch_dev, wl, proj, storage=storage)
+    want = O.fdk(wl.scan, wl.img, proj, f64=True)
+    rep = O.parity_report(got, want)
+    print(f"\n[{g.KIND_NAMES[kind]} storage {storage}] {rep}")
+    assert rep["rmse"] < bound
+    assert plan.info().storage == storage
+
+
+def test_fp64_setup_matches_fp64_oracle(torch_dev):
+    """FP64 per-thread coordinate setup: at least as close to the FP64 oracle as
+    the FP32 setup."""
+    wl = W.small_case(g.FLAT)
+    proj = wl.projections()
+    f32, p32 = _run(torch_dev, wl, proj, setup_precision=g.SETUP_F32)
+    f64, p64 = _run(torch_dev, wl, proj, setup_precision=g.SETUP_F64)
+    assert p64.info().setup_f64 == 1 and p32.info().setup_f64 == 0
+    want = O.fdk(wl.scan, wl.img, proj, f64=True)
+    r32, r64 = O.parity_report(f32, want), O.parity_report(f64, want)
+    print(f"\n[setup f32] {r32}\n[setup f64] {r64}")
+    assert r64["rmse"] <= 1.2 * r32["rmse"] + 1e-9
+    assert O.are_equal(f64, O.fdk(wl.scan, wl.img, proj), 1e-5).all()
+
+
+# ---- batching ---------------------------------------------------------------------------
+
+def test_batch_graph_equals_individual_plans(torch_dev):
+    """Uneven scans (fan arc slices of several sizes, cylindrical and flat slabs)
+    in one persistent launch inside a HIP graph give each scan's own result bit
+    for bit, on every graph replay."""
+    torch = torch_dev
+    rng = np.random.default_rng(11)
+    wls = [W.fan_arc_slice(rng, i) for i in range(3)]
+    wls = [W.Workload(w.name, g.circular_scan(g.FAN_ARC_1D, w.scan.nu // 4, w.scan.du * 4, w.scan.num_proj // 4,
+                                              np.float32(2 * math.pi / (w.scan.num_proj // 4)),
+                                              src_det_dist=1040.0, src_iso_dist=570.0),
+                      g.Image(w.img.nx // 4, w.img.ny // 4, w.img.dx * 4, w.img.dy * 4), w.phantom) for w in wls]
+    wls += [W.small_case(g.CYLINDRICAL, n=40, views=90), W.small_case(g.FLAT, n=48, views=100),
+            W.small_case(g.PARALLEL_2D, n=24, views=60)]
+    projs = [torch.from_numpy(w.projections()).cuda() for w in wls]
+    plans = [g.Plan(w.scan, w.img, bp_kernel=g.BP_DIRECT, z_unroll=8) for w in wls]
+    want = []
+    for pl, p, w in zip(plans, projs, wls):
+        v = torch.empty(w.img.shape, dtype=torch.float32, device="cuda")
+        pl.execute(p, v)
+        want.append(v)
+    torch.cuda.synchronize()
+    want = [v.cpu().numpy() for v in want]
+    batch = g.Batch(plans)
+    vols = [torch.full(w.img.shape, 7.0, dtype=torch.float32, device="cuda") for w in wls]
+    for rep in range(3):
+        for v in vols:
+            v.fill_(float(rep))
+        batch.execute(projs, vols, use_graph=True)
+        torch.cuda.synchronize()
+        for v, ref in zip(vols, want):
+            assert np.array_equal(v.cpu().numpy(), ref)
+    batch.execute(projs, vols, use_graph=False)
+    torch.cuda.synchronize()
+    for v, ref in zip(vols, want):
+        assert np.array_equal(v.cpu().numpy(), ref)
+
+
+def test_autotuner_picks_a_valid_variant_and_keeps_results(torch_dev):
+    torch = torch_dev
+    wl = W.small_case(g.FLAT, n=64, views=120)
+    proj = torch.from_numpy(wl.projections()).cuda()
+    plan = g.Plan(wl.scan, wl.img)
+    v = torch.empty(wl.img.shape, dtype=torch.float32, device="cuda")
+    plan.execute(proj, v)
+    torch.cuda.synchronize()
+    ref = v.cpu().numpy()
+    zu, ms = plan.autotune(v)
+    assert zu in (4, 8, 16, 32) and ms > 0
+    plan.execute(proj, v)
+    torch.cuda.synchronize()
+    assert np.array_equal(v.cpu().numpy(), ref)
+
+
+# ---- serving ------------------------------------------------------------------------------
+
+def test_server_results_identical_alone_and_in_the_mix(torch_dev):
+    """Each request's volume is the same bit for bit when reconstructed alone
+    and when it overlaps other requests on the device."""
+    mix = [W.small_case(k, n=48, views=90) for k in (g.FLAT, g.FAN_ARC_1D, g.CYLINDRICAL, g.PARALLEL_1D,
+                                                     g.CONE_PARALLEL, g.FLAT)]
+    projs = [w.projections() for w in mix]
+    alone = []
+    for w, p in zip(mix, projs):
+        sv = g.Server(num_streams=1, max_inflight=1)
+        out = np.zeros(w.img.shape, np.float32)
+        sv.wait(sv.submit(w.scan, w.img, p, out))
+        sv.close()
+        alone.append(out)
+    sv = g.Server(num_streams=4, max_inflight=4)
+    outs = [np.zeros(w.img.shape, np.float32) for w in mix]
+    tickets = [sv.submit(w.scan, w.img, p, o, reproject=True) for w, p, o in zip(mix, projs, outs)]
+    errs = [sv.wait(t)[1] for t in tickets]
+    st = sv.stats()
+    sv.close()
+    for o, a in zip(outs, alone):
+        assert np.array_equal(o, a)
+    assert all(0.0 <= e < 0.5 for e in errs), errs
+    assert st["completed"] == len(mix) and st["failed"] == 0
+    assert st["peak_inflight"] >= 2
+
+
+def test_server_rejects_requests_over_budget_and_survives_failures(torch_dev):
+    w = W.small_case(g.FLAT, n=32, views=60)
+    p = w.projections()
+    sv = g.Server(hbm_budget_bytes=64 << 20, num_streams=2)
+    big = g.Image(512, 512, 0.1, 0.1, 512, 0.1)
+    with pytest.raises(g.GctError) as e:
+        sv.submit(w.scan, big, p, np.zeros(big.shape, np.float32))
+    assert e.value.status == 6
+    # a stationary trajectory passes submission but fails in the worker; others complete
+    bad_views = w.scan.views.copy()
+    bad_views[:, 5:8] = bad_views[0, 5:8]
+    bad = g.Scan(w.scan.kind, w.scan.nu, w.scan.du, w.scan.nv, w.scan.dv, bad_views)
+    out_ok, out_bad = np.zeros(w.img.shape, np.float32), np.zeros(w.img.shape, np.float32)
+    t_bad = sv.submit(bad, w.img, p, out_bad)
+    t_ok = sv.submit(w.scan, w.img, p, out_ok)
+    with pytest.raises(g.GctError) as e:
+        sv.wait(t_bad)
+    assert e.value.status == 2
+    sv.wait(t_ok)
+    ref = O.fdk(w.scan, w.img, p)
+    assert O.are_equal(out_ok, ref, 1e-5).all()
+    st = sv.stats()
+    assert st["rejected"] == 1 and st["failed"] == 1 and st["completed"] == 1
+    sv.close()
+
+
+# ---- Joseph projector ----------------------------------------------------------------------
+
+@pytest.mark.parametrize("kind", range(7), ids=g.KIND_NAMES)
+def test_joseph_matches_oracle(torch_dev, kind):
+    torch = torch_dev
+    wl = W.small_case(kind, n=40, views=30)
+    vol = g.phantom_rasterize(wl.img, wl.phantom)
+    plan = g.Plan(wl.scan, wl.img)
+    out = torch.empty(wl.scan.proj_shape, dtype=torch.float32, device="cuda")
+    plan.forward_project(torch.from_numpy(vol).cuda(), out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    want = O.joseph(wl.scan, wl.img, vol)
+    rep = O.parity_report(got, want)
+    print(f"\n[joseph {g.KIND_NAMES[kind]}] {rep}")
+    assert rep["max_rel_to_peak"] < 1e-4
+
+
+@pytest.mark.parametrize("kind", range(7), ids=g.KIND_NAMES)
+def test_joseph_approximates_analytic_line_integrals(torch_dev, kind):
+    """Joseph of the rasterised phantom against its exact line integrals."""
+    torch = torch_dev
+    wl = W.small_case(kind, n=64, views=24)
+    vol = g.phantom_rasterize(wl.img, wl.phantom)
+    plan = g.Plan(wl.scan, wl.img)
+    out = torch.empty(wl.scan.proj_shape, dtype=torch.float32, device="cuda")
+    plan.forward_project(torch.from_numpy(vol).cuda(), out)
+    torch.cuda.synchronize()
+    exact = wl.projections()
+    err = np.sqrt(np.mean((out.cpu().numpy() - exact) ** 2)) / np.sqrt(np.mean(exact ** 2))
+    print(f"\n[joseph vs exact {g.KIND_NAMES[kind]}] relative rms {err:.4f}")
+    assert err < 0.05
+
+
+@pytest.mark.parametrize("kind", [g.PARALLEL_1D, g.PARALLEL_2D], ids=lambda k: g.KIND_NAMES[k])
+def test_joseph_adjoint_of_backprojection_for_parallel_beams(torch_dev, kind):
+    """<A x, y> du ~ <x, B y> dx dy with B the unweighted backprojector: the
+    discrete pair are transposes up to interpolation error on smooth data."""
+    torch = torch_dev
+    wl = W.small_case(kind, n=48, views=90)
+    x = g.phantom_rasterize(wl.img, wl.phantom).astype(np.float64)
+    # smooth y: projections of another phantom
+    y = g.phantom_project(wl.scan, g.phantom_random(99, 6, 2 if g.is_1d(kind) else 3, 10.0)).astype(np.float64)
+    plan = g.Plan(wl.scan, wl.img)
+    ax = torch.empty(wl.scan.proj_shape, dtype=torch.float32, device="cuda")
+    plan.forward_project(torch.from_numpy(x.astype(np.float32)).cuda(), ax)
+    plan.load_filtered(torch.from_numpy(y.astype(np.float32)).cuda())
+    by = torch.empty(wl.img.shape, dtype=torch.float32, device="cuda")
+    plan.backproject(by, finish=False)
+    torch.cuda.synchronize()
+    dtheta = float(plan.info().rot_ang_incr)
+    lhs = float((ax.cpu().numpy().astype(np.float64) * y).sum()) * wl.scan.du * dtheta
+    vox = wl.img.dx * wl.img.dy
+    rhs = float((x * by.cpu().numpy().astype(np.float64)).sum()) * vox * dtheta
+    if not g.is_1d(kind):
+        lhs *= wl.scan.dv
+        rhs *= wl.img.dz
+    print(f"\n[adjoint {g.KIND_NAMES[kind]}] {lhs} vs {rhs}")
+    assert lhs == pytest.approx(rhs, rel=0.02)
