@@ -303,12 +303,14 @@ __device__ inline ColSetup setup_column(const BpLaunch &a, uint32_t v, float fx,
 {
     ColSetup s{};
     if constexpr (KIND == GCT_PARALLEL_2D || KIND == GCT_CONE_PARALLEL) {
-        const BpRec &r = a.rec[v];
-        const float col = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
-        s.col = col;
+        // The column is an affine form with a large constant (the detector
+        // centre); evaluating it in FP64 once per column keeps it as accurate
+        // as the reference's FP32 chain t = x cos + y sin, col = t / du + cu.
+        const BpRec64 &r = a.rec64[v];
+        s.col = static_cast<float>(r.u[0] + r.u[1] * static_cast<double>(fx) + r.u[2] * static_cast<double>(fy));
         s.w = 1.0f;
-        s.row0 = fmaf(r.v[3], fz, r.v[0]);
-        s.drow = r.v[3];
+        s.row0 = static_cast<float>(r.v[0] + r.v[3] * static_cast<double>(fz));
+        s.drow = static_cast<float>(r.v[3]);
     } else if constexpr (KIND == GCT_FLAT) {
         if constexpr (F64) {
             const BpRec64 &r = a.rec64[v];
@@ -403,15 +405,18 @@ __device__ inline void bp_tile3d(const BpLaunch &a, uint32_t bx, uint32_t by, ui
     for (uint32_t v = a.first_view; v < vend; ++v) {
         const T *p = base + static_cast<size_t>(v) * view_elems;
         if constexpr (column_invariant<KIND, CIRC>()) {
-            const ColSetup s = setup_column<KIND, F64>(a, v, fx, fy, fz, nuf);
+            // set up at z index 0 and step by the absolute index, so the
+            // result does not depend on how z is split into tiles
+            const ColSetup s = setup_column<KIND, F64>(a, v, fx, fy, 0.0f, nuf);
             const Sample cs = col_sample(s.col, nuf);
 #pragma unroll
             for (int k = 0; k < ZU; ++k) {
-                const float row = fmaf(static_cast<float>(k), s.drow, s.row0);
+                const float zk = fz + static_cast<float>(k);
+                const float row = fmaf(zk, s.drow, s.row0);
                 const float val = bilinear<S>(p, a.pitch, cs, row, nvf);
                 float w = s.w;
                 if constexpr (KIND == GCT_CYLINDRICAL) {
-                    const float nvk = fmaf(static_cast<float>(k), s.dnv, s.nv0);
+                    const float nvk = fmaf(zk, s.dnv, s.nv0);
                     w = s.w / fmaf(nvk, nvk, s.rho2);
                 }
                 acc[k] = fmaf(val, w, acc[k]);
@@ -528,9 +533,9 @@ __device__ inline void tile_boxes(const BpLaunch &a, uint32_t bx, uint32_t by, u
     const uint32_t cy = min(by * 16u + ((corner & 2u) ? 15u : 0u), a.ny - 1u);
     const uint32_t cz = min(bz * ZU + ((corner & 4u) ? static_cast<uint32_t>(ZU - 1) : 0u), a.nz - 1u);
     const float nuf = static_cast<float>(a.nu);
-    const ColSetup s = setup_column<KIND, false>(a, vb0 + j, static_cast<float>(cx), static_cast<float>(cy),
-                                                 static_cast<float>(cz), nuf);
-    float cmin = s.col, cmax = s.col, rmin = s.row0, rmax = s.row0;
+    const ColSetup s = setup_column<KIND, false>(a, vb0 + j, static_cast<float>(cx), static_cast<float>(cy), 0.0f, nuf);
+    const float row = fmaf(static_cast<float>(cz), s.drow, s.row0);
+    float cmin = s.col, cmax = s.col, rmin = row, rmax = row;
 #pragma unroll
     for (int off = 1; off < 8; off <<= 1) {
         cmin = fminf(cmin, __shfl_xor(cmin, off));
@@ -600,7 +605,7 @@ __global__ __launch_bounds__(256) void bp3d_lds_kernel(BpLaunch a)
         if (active) {
             for (uint32_t j = 0; j < nvb; ++j) {
                 const uint32_t v = vb0 + j;
-                const ColSetup s = setup_column<KIND, F64>(a, v, fx, fy, fz, nuf);
+                const ColSetup s = setup_column<KIND, F64>(a, v, fx, fy, 0.0f, nuf);
                 const Box b = boxes[j];
                 if (b.ncols != 0) {
                     const float *tl = tile + j * (TR * kTP);
@@ -611,7 +616,8 @@ __global__ __launch_bounds__(256) void bp3d_lds_kernel(BpLaunch a)
                     const float rlo = static_cast<float>(b.rlo), rhi = static_cast<float>(b.rlo + b.nrows - 2);
 #pragma unroll
                     for (int k = 0; k < ZU; ++k) {
-                        const float row = fminf(fmaxf(fmaf(static_cast<float>(k), s.drow, s.row0), rlo), rhi);
+                        const float zk = fz + static_cast<float>(k);
+                        const float row = fminf(fmaxf(fmaf(zk, s.drow, s.row0), rlo), rhi);
                         const float rf = floorf(row);
                         const float beta = row - rf;
                         const int idx = (static_cast<int>(rf) - b.rlo) * kTP + ci;
@@ -622,7 +628,7 @@ __global__ __launch_bounds__(256) void bp3d_lds_kernel(BpLaunch a)
                         const float val = fmaf(beta, bot - top, top);
                         float w = s.w;
                         if constexpr (KIND == GCT_CYLINDRICAL) {
-                            const float nvk = fmaf(static_cast<float>(k), s.dnv, s.nv0);
+                            const float nvk = fmaf(zk, s.dnv, s.nv0);
                             w = s.w / fmaf(nvk, nvk, s.rho2);
                         }
                         acc[k] = fmaf(val, w, acc[k]);
@@ -632,11 +638,12 @@ __global__ __launch_bounds__(256) void bp3d_lds_kernel(BpLaunch a)
                     const Sample cs = col_sample(s.col, nuf);
 #pragma unroll
                     for (int k = 0; k < ZU; ++k) {
-                        const float row = fmaf(static_cast<float>(k), s.drow, s.row0);
+                        const float zk = fz + static_cast<float>(k);
+                        const float row = fmaf(zk, s.drow, s.row0);
                         const float val = bilinear<S>(p, a.pitch, cs, row, nvf);
                         float w = s.w;
                         if constexpr (KIND == GCT_CYLINDRICAL) {
-                            const float nvk = fmaf(static_cast<float>(k), s.dnv, s.nv0);
+                            const float nvk = fmaf(zk, s.dnv, s.nv0);
                             w = s.w / fmaf(nvk, nvk, s.rho2);
                         }
                         acc[k] = fmaf(val, w, acc[k]);

@@ -244,7 +244,7 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
     std::vector<FiltRec> frec;
     build_records(*p, rec, rec64, frec);
     upload(*p, p->d_brec, rec.data(), rec.size() * sizeof(BpRec));
-    if (p->setup_f64) upload(*p, p->d_brec64, rec64.data(), rec64.size() * sizeof(BpRec64));
+    upload(*p, p->d_brec64, rec64.data(), rec64.size() * sizeof(BpRec64));
     upload(*p, p->d_frec, frec.data(), frec.size() * sizeof(FiltRec));
     upload(*p, p->d_views, p->views.data(), p->views.size() * sizeof(gct_proj_geom));
 

@@ -88,8 +88,11 @@ def test_filter_stage_parity(torch_dev, kind):
     t32, _ = O.ramp_taps(kind in (g.FAN_ARC_1D, g.CYLINDRICAL), wl.scan.nu, wl.scan.du)
     want = O.filter_rows(t32, weighted)
     got = out.cpu().numpy()
-    _report(f"filter {g.KIND_NAMES[kind]}", got, want)
-    assert O.are_equal(got, want, EPS).all()
+    rep = _report(f"filter {g.KIND_NAMES[kind]}", got, want)
+    # FP32 FFT error scales with each row's magnitude, not with each sample:
+    # bound it relative to the peak (arc kernels carry a 1/dgamma scale, so
+    # their filtered values are ~100x larger than the voxel values they make)
+    assert rep["max_rel_to_peak"] <= 1e-5
 
 
 @pytest.mark.parametrize("kind", [g.PARALLEL_1D, g.FAN_ARC_1D, g.PARALLEL_2D, g.CYLINDRICAL, g.FLAT],
@@ -148,12 +151,13 @@ def test_one_shot_high_low_memory_identical(torch_dev, mode):
 @pytest.mark.parametrize("kind", KINDS, ids=g.KIND_NAMES)
 def test_lds_and_direct_kernels_identical(torch_dev, kind):
     """The LDS-staged and the direct backprojector read the same taps and run
-    the same arithmetic per update, so they agree bit for bit."""
+    the same arithmetic per update, and every voxel's row coordinate is stepped
+    from z index 0, so they agree bit for bit whatever the z unroll."""
     torch = torch_dev
     wl = W.small_case(kind)
     proj = torch.from_numpy(wl.projections()).cuda()
     out = []
-    for bk, zu in ((g.BP_DIRECT, 8), (g.BP_LDS, 16), (g.BP_LDS, 32)):
+    for bk, zu in ((g.BP_DIRECT, 8), (g.BP_DIRECT, 4), (g.BP_LDS, 16), (g.BP_LDS, 32)):
         if g.is_1d(kind) and zu == 32:
             continue
         plan = g.Plan(wl.scan, wl.img, bp_kernel=bk, z_unroll=0 if g.is_1d(kind) else zu)
