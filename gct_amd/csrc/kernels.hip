@@ -259,7 +259,7 @@ struct Sample {
 // which reproduces the border-zero texture (nd_array_texture.cuh:43-50).
 __device__ inline Sample col_sample(float col, float nu)
 {
-    const float c = fminf(fmaxf(col, -1.0f), nu);
+    const float c = __builtin_amdgcn_fmed3f(col, -1.0f, nu);
     const float f = floorf(c);
     return Sample{static_cast<uint32_t>(static_cast<int>(f) + 1), c - f};
 }
@@ -268,7 +268,7 @@ template <int S>
 __device__ inline float bilinear(const typename Stor<S>::T *__restrict__ p, uint32_t pitch, Sample cs, float row,
                                  float nvf)
 {
-    const float r = fminf(fmaxf(row, -1.0f), nvf);
+    const float r = __builtin_amdgcn_fmed3f(row, -1.0f, nvf);
     const float fr = floorf(r);
     const float beta = r - fr;
     const uint32_t off = static_cast<uint32_t>(static_cast<int>(fr) + 1) * pitch + cs.colbase;
@@ -509,15 +509,17 @@ __global__ __launch_bounds__(64) void bp2d_kernel(BpLaunch a)
 //
 // A workgroup owns a 16 x 16 x ZU voxel tile.  For each block of VB views it
 // projects the tile's 8 corner voxels, takes the bounding box of the detector
-// pixels they reach (plus one pixel of margin), copies those padded rows from
-// HBM into LDS (FP16/BF16 storage is widened to FP32 on the way), and every
-// thread then interpolates from LDS with two ds_read2 per update.  Coordinates
-// are clamped to the staged box: the box never extends past the zero padding
-// of the filtered-projection buffer, so the clamp also reproduces the
-// border-zero texture, and it keeps every LDS read inside the box.  A view
-// whose box does not fit the staging area is read from HBM directly.
+// pixels they reach (plus one pixel of margin), and stages that box from HBM
+// into LDS as pairs (p[r][c], p[r][c+1] - p[r][c]) -- value and horizontal
+// difference, FP16/BF16 widened to FP32 -- so each thread's bilinear tap is
+// two 64-bit LDS reads and one FMA per row.  The differences are the same
+// FP32 subtractions the direct kernel performs per update, so both kernels
+// give the same bits.  Coordinates are clamped to the staged box: the box never
+// extends past the zero padding of the filtered-projection buffer, so the
+// clamp also reproduces the border-zero texture, and it keeps every LDS read
+// inside the box.  A view whose box does not fit is read from HBM directly.
 
-constexpr int kTP = 64;   // staged columns per row (LDS pitch)
+constexpr int kTP = 48;   // staged (value, difference) pairs per row
 
 struct Box {
     int clo, rlo, ncols, nrows;   // ncols == 0: the view is read from HBM
@@ -552,7 +554,7 @@ __device__ inline void tile_boxes(const BpLaunch &a, uint32_t bx, uint32_t by, u
         const int rlo = max(static_cast<int>(floorf(rmin)) - 1, -1);
         const int rhi = min(static_cast<int>(floorf(rmax)) + 2, static_cast<int>(a.nv) + 1);
         const int nc = chi - clo + 1, nr = rhi - rlo + 1;
-        if (nc >= 2 && nr >= 2 && nc <= kTP && nr <= TR) b = Box{clo, rlo, nc, nr};
+        if (nc >= 2 && nr >= 2 && nc - 1 <= kTP && nr <= TR) b = Box{clo, rlo, nc, nr};
     }
     boxes[j] = b;
 }
@@ -561,7 +563,7 @@ template <int KIND, int S, bool F64, int ZU, int TR, int VB>
 __global__ __launch_bounds__(256) void bp3d_lds_kernel(BpLaunch a)
 {
     using T = typename Stor<S>::T;
-    __shared__ float tile[VB * TR * kTP];
+    __shared__ float2 tile[VB * TR * kTP];
     __shared__ Box boxes[VB];
     const uint32_t tid = threadIdx.x;
     const uint32_t nbx = (a.nx + 15u) / 16u, nby = (a.ny + 15u) / 16u;
@@ -593,12 +595,17 @@ __global__ __launch_bounds__(256) void bp3d_lds_kernel(BpLaunch a)
             const Box b = boxes[j];
             if (b.ncols == 0) continue;
             const T *p = base + static_cast<size_t>(vb0 + j) * view_elems;
-            float *tl = tile + j * (TR * kTP);
-            const int c = static_cast<int>(tid & 63u);
-            if (c < b.ncols) {
-                for (int r = static_cast<int>(tid >> 6); r < b.nrows; r += 4)
-                    tl[r * kTP + c] = Stor<S>::ld(p, static_cast<uint32_t>(b.rlo + r + 1) * a.pitch +
-                                                          static_cast<uint32_t>(b.clo + c + 1));
+            float2 *tl = tile + j * (TR * kTP);
+            const int m = b.ncols - 1;            // staged pairs per row
+            const bool narrow = m <= 32;          // two rows per wave
+            const int c = narrow ? static_cast<int>(tid & 31u) : static_cast<int>(tid & 63u);
+            const int rstep = narrow ? 8 : 4;
+            if (c < m) {
+                for (int r = narrow ? static_cast<int>(tid >> 5) : static_cast<int>(tid >> 6); r < b.nrows; r += rstep) {
+                    const uint32_t g = static_cast<uint32_t>(b.rlo + r + 1) * a.pitch + static_cast<uint32_t>(b.clo + c + 1);
+                    const float v0 = Stor<S>::ld(p, g), v1 = Stor<S>::ld(p, g + 1u);
+                    tl[r * kTP + c] = make_float2(v0, v1 - v0);
+                }
             }
         }
         __syncthreads();
@@ -608,23 +615,28 @@ __global__ __launch_bounds__(256) void bp3d_lds_kernel(BpLaunch a)
                 const ColSetup s = setup_column<KIND, F64>(a, v, fx, fy, 0.0f, nuf);
                 const Box b = boxes[j];
                 if (b.ncols != 0) {
-                    const float *tl = tile + j * (TR * kTP);
-                    const float cl = fminf(fmaxf(s.col, static_cast<float>(b.clo)), static_cast<float>(b.clo + b.ncols - 2));
+                    const float cl = __builtin_amdgcn_fmed3f(s.col, static_cast<float>(b.clo),
+                                                             static_cast<float>(b.clo + b.ncols - 2));
                     const float cf = floorf(cl);
                     const float alpha = cl - cf;
-                    const int ci = static_cast<int>(cf) - b.clo;
+                    // Pair (r, c) of the box sits at byte 8 (j TR kTP + (r - rlo) kTP + (c - clo)).
+                    // The per-view terms fold into one base; the row term is added in float,
+                    // which is exact because every offset is an integer below 2^24.
+                    // (signed: the base is negative when the box starts below row 0 of view j's slot)
+                    const int ebase = static_cast<int>(j) * (TR * kTP) + (static_cast<int>(cf) - b.clo) - b.rlo * kTP;
+                    const float obase = static_cast<float>(8 * ebase);
                     const float rlo = static_cast<float>(b.rlo), rhi = static_cast<float>(b.rlo + b.nrows - 2);
 #pragma unroll
                     for (int k = 0; k < ZU; ++k) {
                         const float zk = fz + static_cast<float>(k);
-                        const float row = fminf(fmaxf(fmaf(zk, s.drow, s.row0), rlo), rhi);
+                        const float row = __builtin_amdgcn_fmed3f(fmaf(zk, s.drow, s.row0), rlo, rhi);
                         const float rf = floorf(row);
                         const float beta = row - rf;
-                        const int idx = (static_cast<int>(rf) - b.rlo) * kTP + ci;
-                        const float t0 = tl[idx], t1 = tl[idx + 1];
-                        const float b0 = tl[idx + kTP], b1 = tl[idx + kTP + 1];
-                        const float top = fmaf(alpha, t1 - t0, t0);
-                        const float bot = fmaf(alpha, b1 - b0, b0);
+                        const uint32_t off = static_cast<uint32_t>(fmaf(rf, static_cast<float>(8 * kTP), obase));
+                        const float2 *q = reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(tile) + off);
+                        const float2 tp = q[0], bt = q[kTP];
+                        const float top = fmaf(alpha, tp.y, tp.x);
+                        const float bot = fmaf(alpha, bt.y, bt.x);
                         const float val = fmaf(beta, bot - top, top);
                         float w = s.w;
                         if constexpr (KIND == GCT_CYLINDRICAL) {
@@ -1042,8 +1054,8 @@ void launch_bp3d_lds(const BpLaunch &a, hipStream_t s)
     const uint32_t nbx = (a.nx + 15u) / 16u, nby = (a.ny + 15u) / 16u;
     const uint32_t zu = (a.z_unroll == 32u) ? 32u : 16u;
     const uint32_t blocks = nbx * nby * ((a.nz + zu - 1u) / zu);
-    if (zu == 32u) hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 32, 80, 2>), dim3(blocks), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 48, 4>), dim3(blocks), dim3(256), 0, s, a);
+    if (zu == 32u) hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 32, 64, 2>), dim3(blocks), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 3>), dim3(blocks), dim3(256), 0, s, a);
 }
 
 template <int KIND, int S>
