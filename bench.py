@@ -1,13 +1,31 @@
 #!/usr/bin/env python3
-"""FDK benchmark on one MI355X.
+"""FDK benchmark on one MI355X.  Prints one JSON line.
 
-A step is one plain FDK reconstruction of the flat-cone workload (256^3 at
-0.25 mm, 512 views over 360 deg, 512 x 384 detector at 0.4 mm, SOD 500 mm,
-SDD 1000 mm, Shepp-Logan-windowed ramp, 12 seeded ellipsoids with Poisson
-noise at I0 = 1e5): fused weighting + FFT ramp filter, then backprojection,
-with the projections already resident in HBM.  Prints one JSON line.
+Sections (all on by default; `--sections flat` runs the first only):
 
-    python bench.py [--steps K] [--warmup W] [--no-cpu-baseline]
+flat      The headline number.  A step is one plain FDK reconstruction of the
+          flat-cone workload (256^3 at 0.25 mm, 512 views over 360 deg,
+          512 x 384 detector at 0.4 mm, SOD 500 mm, SDD 1000 mm,
+          Shepp-Logan-windowed ramp, 12 seeded ellipsoids with Poisson noise
+          at I0 = 1e5): fused weighting + FFT ramp filter, then
+          backprojection, projections already resident in HBM.  The plan is
+          autotuned once before the warm-up.  `value` is voxel x view updates
+          per second over the timed steps; the backprojection kernel's
+          roofline is measured with HIP events on its stream.
+batched   The uneven batch: 64 fan-arc slices of seeded sizes plus 16
+          cylindrical 512 x 512 x 64 slabs (672 x 64 detector, 1152 views).
+          Plain = each scan's FDK in turn, waiting for each (one scan per
+          call); batched = every filter plus one persistent backprojection
+          launch per kernel group, replayed from one HIP graph.
+served    The serving stream: 32 seeded requests of mixed geometry and size,
+          each FDK + Joseph reprojection + reprojection-error check, host
+          buffers in and out.  Plain = the same pipeline one request at a
+          time (one stream, one in flight); served = the continuous-admission
+          server (4 streams, HBM-footprint admission), first saturated (all
+          requests queued at once: capacity), then with Poisson arrivals at
+          80 % of that capacity (latency).
+
+    python bench.py [--steps K] [--warmup W] [--no-cpu-baseline] [--sections flat,batched,served]
 """
 from __future__ import annotations
 
@@ -16,6 +34,7 @@ import json
 import os
 import statistics
 import sys
+import threading
 import time
 from pathlib import Path
 
@@ -42,7 +61,6 @@ def cpu_baseline(wl, proj, gpu_vol, nthreads):
     z slices, scaled to the whole volume.  Also reports how the GPU volume
     agrees with the oracle on those slices."""
     sys.path.insert(0, str(ROOT / "tests"))
-    import numpy as np
     import oracle_lib as O
 
     scan, img = wl.scan, wl.img
@@ -72,39 +90,9 @@ def cpu_baseline(wl, proj, gpu_vol, nthreads):
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="only run a 1 GiB device copy 5 times (reference kernel for HBM byte counters)")
-    a = ap.parse_args()
-    if a.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (the FDK path here "
-              "runs on one device; DESIGN.md describes how it would shard).", file=sys.stderr)
-        sys.exit(2)
+# ---- flat-cone plain FDK (the headline) -----------------------------------------
 
-    import numpy as np
-    import torch
-
-    import gct_amd as g
-    from gct_amd import workloads as W
-
-    if not torch.cuda.is_available():
-        print("no GPU visible", file=sys.stderr)
-        sys.exit(1)
-    if a.copy_calibration:
-        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
-        y = torch.empty_like(x)
-        for _ in range(5):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes_per_launch": {"read": x.numel() * 4, "write": y.numel() * 4}}))
-        return
-
+def run_flat(a, g, W, torch, np):
     wl = W.flat_cone_256()
     t = time.perf_counter()
     proj = wl.projections(a.threads)
@@ -112,8 +100,11 @@ def main():
     p = torch.from_numpy(proj).cuda()
     v = torch.empty(wl.img.shape, dtype=torch.float32, device="cuda")
     plan = g.Plan(wl.scan, wl.img, window=wl.window)
-    info = plan.info()
     stream = torch.cuda.current_stream()
+    plan.filter(p, stream=stream)
+    torch.cuda.synchronize()
+    tuned_zu, tuned_ms = plan.autotune(v)
+    info = plan.info()
 
     for _ in range(a.warmup):
         plan.execute(p, v, stream=stream)
@@ -142,18 +133,19 @@ def main():
 
     achieved_tflops = F_ALG_FLAT * upd / (bp_mean * 1e-3) / 1e12
     alg_bytes = wl.scan.num_proj * wl.scan.nv * wl.scan.nu * 4 + wl.img.nvox * 4
-    traffic = None
+    traffic, kernel = None, None
     tfile = ROOT / "profiles" / "bp_flat_cone_256_traffic.json"
     if tfile.exists():
         try:
-            traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
-        except Exception:
+            tj = json.loads(tfile.read_text())
+            traffic, kernel = tj.get("hbm_bytes_per_launch"), tj.get("kernel")
+        except (OSError, ValueError):
             traffic = None
 
     vol = v.cpu().numpy()
     truth = g.phantom_rasterize(wl.img, wl.phantom)
     rmse_truth = float(np.sqrt(np.mean((vol - truth) ** 2)))
-
+    variant = {g.BP_LDS: "lds", g.BP_DIRECT: "direct"}.get(info.bp_kernel, str(info.bp_kernel))
     out = {
         "metric": "backprojection GUPS + p50/p99 recon latency, served vs plain FDK, at fixed RMSE",
         "value": round(gups, 3),
@@ -172,7 +164,9 @@ def main():
                         "SOD 500 mm, SDD 1000 mm, Shepp-Logan ramp, plain FDK (one scan per launch sequence)",
             "updates_per_step": upd,
             "storage": "f32",
+            "bp_kernel": variant,
             "z_unroll": info.z_unroll,
+            "autotune_best_bp_ms": round(tuned_ms, 4),
             "fft_len": info.fft_len,
         },
         "bp_gups": round(upd / (bp_mean * 1e-3) / 1e9, 3),
@@ -182,7 +176,7 @@ def main():
         "rmse_vs_phantom": rmse_truth,
         "roofline": {
             "bound": "valu",
-            "kernel": "bp3d_kernel<FLAT, f32>",
+            "kernel": kernel or f"bp3d_{variant}_kernel<FLAT, f32, ZU={info.z_unroll}>",
             "achieved": round(achieved_tflops, 3),
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
@@ -194,6 +188,213 @@ def main():
         },
         "input_generation_s": round(t_gen, 2),
     }
+    del p, v
+    plan.close()
+    return out, wl, proj, vol
+
+
+# ---- uneven batch: plain per-scan calls vs one graph ------------------------------
+
+def run_batched(a, g, W, torch, np):
+    wls = W.uneven_batch()
+    stream = torch.cuda.current_stream()
+    t = time.perf_counter()
+    projs = [w.projections_device() for w in wls]
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t
+    vols = [torch.empty(w.img.shape, dtype=torch.float32, device="cuda") for w in wls]
+    plans = [g.Plan(w.scan, w.img) for w in wls]
+    upd = sum(w.updates for w in wls)
+
+    def plain():
+        for pl, p, v in zip(plans, projs, vols):
+            pl.execute(p, v, stream=stream)
+            stream.synchronize()
+
+    def plain_async():
+        for pl, p, v in zip(plans, projs, vols):
+            pl.execute(p, v, stream=stream)
+        stream.synchronize()
+
+    plain()
+    want = [v.clone() for v in vols]
+    batch = g.Batch(plans)
+    batch.execute(projs, vols, use_graph=True, stream=stream)
+    stream.synchronize()
+    same = all(bool(torch.equal(v, w)) for v, w in zip(vols, want))
+    del want
+
+    reps = max(1, min(a.steps, 3))
+
+    def timed(fn):
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return statistics.median(ts)
+
+    t_plain = timed(plain)
+    t_async = timed(plain_async)
+    t_batch = timed(lambda: batch.execute(projs, vols, use_graph=True, stream=stream))
+    n_slices = sum(1 for w in wls if g.is_1d(w.scan.kind))
+    out = {
+        "workload": f"{n_slices} fan-arc slices (channels U{{300..900}}, views U{{360..1200}}, images U{{256..768}}^2) "
+                    f"+ {len(wls) - n_slices} cylindrical slabs (672x64 det, 1152 views, 512x512x64)",
+        "updates": upd,
+        "plain_s": round(t_plain, 4),
+        "plain_gups": round(upd / t_plain / 1e9, 2),
+        "plain_no_wait_s": round(t_async, 4),
+        "batched_graph_s": round(t_batch, 4),
+        "batched_gups": round(upd / t_batch / 1e9, 2),
+        "batched_vs_plain": round(t_plain / t_batch, 3),
+        "batched_identical_to_plain": same,
+        "input_generation_s": round(t_gen, 2),
+    }
+    batch.close()
+    for pl in plans:
+        pl.close()
+    return out
+
+
+# ---- serving stream ------------------------------------------------------------
+
+def run_served(a, g, W, torch, np):
+    reqs = W.serving_mix()
+    t = time.perf_counter()
+    host_proj, host_vol = [], []
+    for w in reqs:
+        d = w.projections_device()
+        hp = torch.empty(w.scan.proj_shape, dtype=torch.float32, pin_memory=True)
+        hp.copy_(d)
+        host_proj.append(hp)
+        host_vol.append(torch.empty(w.img.shape, dtype=torch.float32, pin_memory=True))
+        del d
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t
+    pn = [p.numpy() for p in host_proj]
+    vn = [v.numpy() for v in host_vol]
+    upd = sum(w.updates for w in reqs)
+
+    def closed_loop(server):
+        t0 = time.perf_counter()
+        tickets = [server.submit(w.scan, w.img, p, v, reproject=True) for w, p, v in zip(reqs, pn, vn)]
+        res = [server.wait(tk) for tk in tickets]
+        return time.perf_counter() - t0, res
+
+    plain = g.Server(num_streams=1, max_inflight=1)
+    closed_loop(plain)                      # warm-up (pool, kernels)
+    t_plain, res_plain = closed_loop(plain)
+    plain.close()
+    plain_vols = [v.copy() for v in vn[:4]]
+
+    served = g.Server()
+    closed_loop(served)
+    t_served, res_served = closed_loop(served)
+    same = all(np.array_equal(v, w) for v, w in zip(vn[:4], plain_vols))
+
+    # Poisson arrivals at 80 % of the measured served capacity
+    qps = 0.8 * len(reqs) / t_served
+    rng = np.random.default_rng(1005)
+    gaps = rng.exponential(1.0 / qps, len(reqs))
+    tickets = [None] * len(reqs)
+
+    def feeder():
+        t0 = time.perf_counter()
+        due = 0.0
+        for i, (w, p, v) in enumerate(zip(reqs, pn, vn)):
+            due += gaps[i]
+            dt = t0 + due - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            tickets[i] = served.submit(w.scan, w.img, p, v, reproject=True)
+
+    t0 = time.perf_counter()
+    th = threading.Thread(target=feeder)
+    th.start()
+    th.join()
+    res_poisson = [served.wait(tk) for tk in tickets]
+    t_poisson = time.perf_counter() - t0
+    stats = served.stats()
+    served.close()
+
+    lat_plain = [r[0] for r in res_plain]
+    lat_sat = [r[0] for r in res_served]
+    lat_p = [r[0] for r in res_poisson]
+    errs = [r[1] for r in res_poisson]
+    kinds = {}
+    for w in reqs:
+        kinds[g.KIND_NAMES[w.scan.kind]] = kinds.get(g.KIND_NAMES[w.scan.kind], 0) + 1
+    return {
+        "workload": f"{len(reqs)} requests (seeded mix {kinds}), each FDK + Joseph reprojection + residual, "
+                    "pinned host buffers in/out",
+        "updates": upd,
+        "plain_s": round(t_plain, 4),
+        "plain_gups": round(upd / t_plain / 1e9, 2),
+        "plain_service_ms": {"p50": round(_percentile(lat_plain, 0.5), 2),
+                             "p99": round(_percentile(lat_plain, 0.99), 2)},
+        "served_capacity_s": round(t_served, 4),
+        "served_gups": round(upd / t_served / 1e9, 2),
+        "served_vs_plain": round(t_plain / t_served, 3),
+        "served_saturated_latency_ms": {"p50": round(_percentile(lat_sat, 0.5), 2),
+                                        "p99": round(_percentile(lat_sat, 0.99), 2)},
+        "poisson_qps": round(qps, 3),
+        "poisson_latency_ms": {"p50": round(_percentile(lat_p, 0.5), 2), "p99": round(_percentile(lat_p, 0.99), 2)},
+        "poisson_gups": round(upd / t_poisson / 1e9, 2),
+        "reprojection_rel_err": {"median": round(statistics.median(errs), 5), "max": round(max(errs), 5)},
+        "identical_alone_and_served": same,
+        "peak_device_bytes": stats["peak_device_bytes"],
+        "peak_inflight": stats["peak_inflight"],
+        "input_generation_s": round(t_gen, 2),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sections", default="flat,batched,served")
+    ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy 5 times (reference kernel for HBM byte counters)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (the FDK path here "
+              "runs on one device; DESIGN.md describes how it would shard).", file=sys.stderr)
+        sys.exit(2)
+    sections = {s.strip() for s in a.sections.split(",") if s.strip()}
+
+    import numpy as np
+    import torch
+
+    import gct_amd as g
+    from gct_amd import workloads as W
+
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    if a.copy_calibration:
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_per_launch": {"read": x.numel() * 4, "write": y.numel() * 4}}))
+        return
+
+    out, wl, proj, vol = run_flat(a, g, W, torch, np)
+    if "batched" in sections:
+        print("bench: batched section", file=sys.stderr, flush=True)
+        out["batched"] = run_batched(a, g, W, torch, np)
+        torch.cuda.empty_cache()
+    if "served" in sections:
+        print("bench: served section", file=sys.stderr, flush=True)
+        out["served"] = run_served(a, g, W, torch, np)
+        torch.cuda.empty_cache()
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, proj, vol, a.threads)
     print(json.dumps(out), flush=True)

@@ -162,6 +162,8 @@ def _declare(L: C.CDLL) -> None:
                                     C.c_double, P]),
         "gct_phantom_project": (st, [C.POINTER(CTGeom), C.c_int, P, C.c_uint32, C.c_double, C.c_uint64, P, C.c_int]),
         "gct_phantom_rasterize": (st, [C.POINTER(ImageGeom), P, C.c_uint32, P]),
+        "gct_phantom_project_device": (st, [C.POINTER(CTGeom), C.c_int, P, C.c_uint32, C.c_double, C.c_uint64, P, P]),
+        "gct_phantom_rasterize_device": (st, [C.POINTER(ImageGeom), P, C.c_uint32, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name, None)
@@ -401,6 +403,32 @@ def phantom_rasterize(img: Image, params: np.ndarray) -> np.ndarray:
     vol = np.zeros(img.shape, np.float32)
     check(lib().gct_phantom_rasterize(C.byref(img.c()), params.ctypes.data, params.shape[0], vol.ctypes.data))
     return vol
+
+
+def phantom_project_device(scan: Scan, params: np.ndarray, out, i0: float = 0.0, noise_seed: int = 0, stream=None):
+    """Projections of a phantom written into the device tensor `out`
+    (shape scan.proj_shape, float32) by the GPU generator."""
+    params = np.ascontiguousarray(params, np.float64)
+    if tuple(out.shape) != tuple(scan.proj_shape) or out.dtype != _torch().float32 or not out.is_contiguous():
+        raise ValueError(f"out must be a contiguous float32 tensor of shape {scan.proj_shape}")
+    dim = 2 if is_1d(scan.kind) else 3
+    check(lib().gct_phantom_project_device(C.byref(scan.c()), dim, params.ctypes.data, params.shape[0], i0,
+                                           noise_seed, _ptr(out), _stream(stream)))
+    return out
+
+
+def phantom_rasterize_device(img: Image, params: np.ndarray, out, stream=None):
+    params = np.ascontiguousarray(params, np.float64)
+    if tuple(out.shape) != tuple(img.shape) or out.dtype != _torch().float32 or not out.is_contiguous():
+        raise ValueError(f"out must be a contiguous float32 tensor of shape {img.shape}")
+    check(lib().gct_phantom_rasterize_device(C.byref(img.c()), params.ctypes.data, params.shape[0], _ptr(out),
+                                             _stream(stream)))
+    return out
+
+
+def _torch():
+    import torch
+    return torch
 
 
 def device_count() -> int:

@@ -162,13 +162,29 @@ void launch_pack_filtered(const float *src, void *filt, int storage, uint32_t nu
 void launch_unpack_filtered(const void *filt, float *dst, int storage, uint32_t nu, uint32_t nv,
                             uint32_t nviews, uint32_t pitch, uint32_t rows_pv, hipStream_t s);
 
+// z-extent of one work item of the persistent batched backprojection: LDS
+// tiles for circular scans (the parallel kinds always are), direct tiles for
+// general trajectories.
+constexpr uint32_t kBatchLdsZU = 16, kBatchDirectZU = 8;
+__host__ __device__ inline uint32_t batch_tile_depth(int kind, int circular)
+{
+    return (kind == GCT_PARALLEL_2D || kind == GCT_CONE_PARALLEL || circular) ? kBatchLdsZU : kBatchDirectZU;
+}
+
 // One work item of the persistent batched backprojection: a z-block of one scan.
 struct BatchScan {
     BpLaunch bp;                  // per-scan parameters (views: all)
     uint32_t tiles_x, tiles_y, tiles_z;
     uint32_t tile_begin;          // prefix sum over scans
 };
-void launch_backproject_batch(const BatchScan *scans_dev, uint32_t num_scans, uint32_t total_tiles,
-                              uint32_t *counter, hipStream_t s);
+// Scans of a batch that share one specialised persistent kernel: a contiguous
+// range of the batch's scan table, tile_begin counted from the range start.
+struct BatchGroup {
+    int kind, storage;
+    bool lds;
+    uint32_t first_scan, num_scans, total_tiles;
+};
+void launch_backproject_batch_group(const BatchScan *scans_dev, const BatchGroup &g, uint32_t *counter, hipStream_t s);
+void launch_zero_counters(uint32_t *counters, uint32_t n, hipStream_t s);
 
 }  // namespace gct

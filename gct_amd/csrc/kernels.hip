@@ -559,17 +559,15 @@ __device__ inline void tile_boxes(const BpLaunch &a, uint32_t bx, uint32_t by, u
     boxes[j] = b;
 }
 
+// One 16 x 16 x ZU tile (bx, by, bz) of a circular 3-D scan; every thread of
+// the 256-thread workgroup calls it (it holds workgroup barriers).  `tile`
+// and `boxes` are the workgroup's LDS staging area of VB * TR * kTP pairs and
+// VB boxes.
 template <int KIND, int S, bool F64, int ZU, int TR, int VB>
-__global__ __launch_bounds__(256) void bp3d_lds_kernel(BpLaunch a)
+__device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint32_t bz, uint32_t tid, float2 *tile,
+                              Box *boxes)
 {
     using T = typename Stor<S>::T;
-    __shared__ float2 tile[VB * TR * kTP];
-    __shared__ Box boxes[VB];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t nbx = (a.nx + 15u) / 16u, nby = (a.ny + 15u) / 16u;
-    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
-    const uint32_t bx = t % nbx, rest = t / nbx;
-    const uint32_t by = rest % nby, bz = rest / nby;
     const uint32_t ix = bx * 16u + (tid & 15u), iy = by * 16u + (tid >> 4), iz0 = bz * ZU;
     const bool active = ix < a.nx && iy < a.ny;
     const uint32_t zn = min(static_cast<uint32_t>(ZU), a.nz - iz0);
@@ -672,6 +670,17 @@ __global__ __launch_bounds__(256) void bp3d_lds_kernel(BpLaunch a)
         if (static_cast<uint32_t>(k) < zn) vp[k * slice] = a.finish ? acc[k] * sc : acc[k];
 }
 
+template <int KIND, int S, bool F64, int ZU, int TR, int VB>
+__global__ __launch_bounds__(256) void bp3d_lds_kernel(BpLaunch a)
+{
+    __shared__ float2 tile[VB * TR * kTP];
+    __shared__ Box boxes[VB];
+    const uint32_t nbx = (a.nx + 15u) / 16u, nby = (a.ny + 15u) / 16u;
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t bx = t % nbx, rest = t / nbx;
+    bp_tile3d_lds<KIND, S, F64, ZU, TR, VB>(a, bx, rest % nby, rest / nby, threadIdx.x, tile, boxes);
+}
+
 // 2-D images: each block of views has its (padded) detector rows staged whole
 // in LDS; a 16 x 16 pixel tile per workgroup.
 template <int KIND, int S>
@@ -730,47 +739,25 @@ __global__ __launch_bounds__(256) void bp2d_lds_kernel(BpLaunch a, uint32_t vb)
 
 // ---- persistent batched backprojection -------------------------------------
 
-constexpr int kBatchZU = 8;
+constexpr int kBatchZU = static_cast<int>(kBatchDirectZU);   // direct tiles (general trajectories)
+constexpr int kBatchTR = 40, kBatchVB = 3;                    // LDS tiles (circular), depth kBatchLdsZU
 
-template <int S>
-__device__ void batch_tile(const BatchScan &sc, uint32_t local, uint32_t tid)
-{
-    const BpLaunch &a = sc.bp;
-    if (is_1d(a.kind)) {
-        const uint32_t bx = local % sc.tiles_x, by = local / sc.tiles_x;
-        const uint32_t ix = bx * 16u + (tid & 15u), iy = by * 16u + (tid >> 4);
-        switch (a.kind) {
-        case GCT_PARALLEL_1D: bp_tile2d<GCT_PARALLEL_1D, S>(a, ix, iy); break;
-        case GCT_FAN_ARC_1D: bp_tile2d<GCT_FAN_ARC_1D, S>(a, ix, iy); break;
-        default: bp_tile2d<GCT_FAN_LINE_1D, S>(a, ix, iy); break;
-        }
-        return;
-    }
-    const uint32_t bx = local % sc.tiles_x, rest = local / sc.tiles_x;
-    const uint32_t by = rest % sc.tiles_y, bz = rest / sc.tiles_y;
-    switch (a.kind) {
-    case GCT_PARALLEL_2D: bp_tile3d<GCT_PARALLEL_2D, S, true, false, kBatchZU>(a, bx, by, bz, tid); break;
-    case GCT_CONE_PARALLEL: bp_tile3d<GCT_CONE_PARALLEL, S, true, false, kBatchZU>(a, bx, by, bz, tid); break;
-    case GCT_CYLINDRICAL:
-        if (a.circular) bp_tile3d<GCT_CYLINDRICAL, S, true, false, kBatchZU>(a, bx, by, bz, tid);
-        else bp_tile3d<GCT_CYLINDRICAL, S, false, false, kBatchZU>(a, bx, by, bz, tid);
-        break;
-    default:
-        if (a.circular) bp_tile3d<GCT_FLAT, S, true, false, kBatchZU>(a, bx, by, bz, tid);
-        else bp_tile3d<GCT_FLAT, S, false, false, kBatchZU>(a, bx, by, bz, tid);
-        break;
-    }
-}
-
-// Work items are (scan, tile) pairs numbered scan by scan; each block takes
-// items from a device counter until none are left, so the grid drains on its
-// own whatever the mix of scan sizes.
+// Persistent backprojection over the scans of one group (same geometry kind,
+// storage and tile type; a batch launches one such kernel per group).  Work
+// items are (scan, tile) pairs numbered scan by scan; each workgroup takes
+// items from the group's device counter until none are left, so the grid
+// drains on its own whatever the mix of scan sizes.
+template <int KIND, int S, bool LDS>
 __global__ __launch_bounds__(256) void bp_batch_kernel(const BatchScan *__restrict__ scans, uint32_t nscans,
                                                        uint32_t total, uint32_t *counter)
 {
+    constexpr bool k1d = KIND <= GCT_FAN_LINE_1D;
     __shared__ uint32_t item;
+    __shared__ float2 tile[LDS ? kBatchVB * kBatchTR * kTP : 1];
+    __shared__ Box boxes[kBatchVB];
+    const uint32_t tid = threadIdx.x;
     for (;;) {
-        if (threadIdx.x == 0) item = atomicAdd(counter, 1u);
+        if (tid == 0) item = atomicAdd(counter, 1u);
         __syncthreads();
         const uint32_t t = item;
         __syncthreads();
@@ -781,16 +768,27 @@ __global__ __launch_bounds__(256) void bp_batch_kernel(const BatchScan *__restri
             if (scans[mid].tile_begin <= t) lo = mid; else hi = mid;
         }
         const BatchScan &sc = scans[lo];
+        const BpLaunch &a = sc.bp;
         const uint32_t local = t - sc.tile_begin;
-        switch (sc.bp.storage) {
-        case GCT_STORE_F16: batch_tile<GCT_STORE_F16>(sc, local, threadIdx.x); break;
-        case GCT_STORE_BF16: batch_tile<GCT_STORE_BF16>(sc, local, threadIdx.x); break;
-        default: batch_tile<GCT_STORE_F32>(sc, local, threadIdx.x); break;
+        if constexpr (k1d) {
+            const uint32_t bx = local % sc.tiles_x, by = local / sc.tiles_x;
+            bp_tile2d<KIND, S>(a, bx * 16u + (tid & 15u), by * 16u + (tid >> 4));
+        } else {
+            const uint32_t bx = local % sc.tiles_x, rest = local / sc.tiles_x;
+            const uint32_t by = rest % sc.tiles_y, bz = rest / sc.tiles_y;
+            if constexpr (LDS)
+                bp_tile3d_lds<KIND, S, false, static_cast<int>(kBatchLdsZU), kBatchTR, kBatchVB>(a, bx, by, bz, tid,
+                                                                                                 tile, boxes);
+            else
+                bp_tile3d<KIND, S, false, false, kBatchZU>(a, bx, by, bz, tid);
         }
     }
 }
 
-__global__ void zero_counter_kernel(uint32_t *c) { *c = 0u; }
+__global__ void zero_counters_kernel(uint32_t *c, uint32_t n)
+{
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) c[i] = 0u;
+}
 
 // ---- packing between [V][Nv][Nu] and the padded layout ---------------------
 
@@ -1190,15 +1188,57 @@ void launch_residual(const float *a, const float *b, size_t n, double *partials,
     hip_check(hipGetLastError(), "residual");
 }
 
-void launch_backproject_batch(const BatchScan *scans_dev, uint32_t num_scans, uint32_t total_tiles, uint32_t *counter,
-                              hipStream_t s)
+template <int KIND, int S>
+void launch_batch_kind(const BatchScan *scans, const BatchGroup &g, uint32_t *counter, uint32_t grid, hipStream_t s)
+{
+    if constexpr (KIND <= GCT_FAN_LINE_1D) {
+        hipLaunchKernelGGL((bp_batch_kernel<KIND, S, false>), dim3(grid), dim3(256), 0, s, scans, g.num_scans,
+                           g.total_tiles, counter);
+    } else if constexpr (KIND == GCT_FLAT || KIND == GCT_CYLINDRICAL) {
+        if (g.lds)
+            hipLaunchKernelGGL((bp_batch_kernel<KIND, S, true>), dim3(grid), dim3(256), 0, s, scans, g.num_scans,
+                               g.total_tiles, counter);
+        else
+            hipLaunchKernelGGL((bp_batch_kernel<KIND, S, false>), dim3(grid), dim3(256), 0, s, scans, g.num_scans,
+                               g.total_tiles, counter);
+    } else {
+        hipLaunchKernelGGL((bp_batch_kernel<KIND, S, true>), dim3(grid), dim3(256), 0, s, scans, g.num_scans,
+                           g.total_tiles, counter);
+    }
+}
+
+template <int S>
+void launch_batch_S(const BatchScan *scans, const BatchGroup &g, uint32_t *counter, uint32_t grid, hipStream_t s)
+{
+    switch (g.kind) {
+    case GCT_PARALLEL_1D: launch_batch_kind<GCT_PARALLEL_1D, S>(scans, g, counter, grid, s); break;
+    case GCT_FAN_ARC_1D: launch_batch_kind<GCT_FAN_ARC_1D, S>(scans, g, counter, grid, s); break;
+    case GCT_FAN_LINE_1D: launch_batch_kind<GCT_FAN_LINE_1D, S>(scans, g, counter, grid, s); break;
+    case GCT_PARALLEL_2D: launch_batch_kind<GCT_PARALLEL_2D, S>(scans, g, counter, grid, s); break;
+    case GCT_CONE_PARALLEL: launch_batch_kind<GCT_CONE_PARALLEL, S>(scans, g, counter, grid, s); break;
+    case GCT_CYLINDRICAL: launch_batch_kind<GCT_CYLINDRICAL, S>(scans, g, counter, grid, s); break;
+    default: launch_batch_kind<GCT_FLAT, S>(scans, g, counter, grid, s); break;
+    }
+}
+
+void launch_zero_counters(uint32_t *counters, uint32_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, counters, n);
+    hip_check(hipGetLastError(), "zero counters");
+}
+
+void launch_backproject_batch_group(const BatchScan *scans_dev, const BatchGroup &g, uint32_t *counter, hipStream_t s)
 {
     int dev = 0, cus = 256;
     hip_check(hipGetDevice(&dev), "hipGetDevice");
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 256;
-    const uint32_t grid = static_cast<uint32_t>(cus) * 8u;
-    hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, s, counter);
-    hipLaunchKernelGGL(bp_batch_kernel, dim3(grid), dim3(256), 0, s, scans_dev, num_scans, total_tiles, counter);
+    const uint32_t grid = std::max(1u, std::min(static_cast<uint32_t>(cus) * 8u, g.total_tiles));
+    const BatchScan *scans = scans_dev + g.first_scan;
+    switch (g.storage) {
+    case GCT_STORE_F16: launch_batch_S<GCT_STORE_F16>(scans, g, counter, grid, s); break;
+    case GCT_STORE_BF16: launch_batch_S<GCT_STORE_BF16>(scans, g, counter, grid, s); break;
+    default: launch_batch_S<GCT_STORE_F32>(scans, g, counter, grid, s); break;
+    }
     hip_check(hipGetLastError(), "batched backprojection");
 }
 

@@ -19,6 +19,7 @@
 #include <mutex>
 #include <numbers>
 #include <tuple>
+#include <tuple>
 
 namespace gct {
 
@@ -731,12 +732,17 @@ extern "C" gct_status gct_plan_autotune(gct_plan *p, float *vol, uint32_t *best_
 // ---- batches ------------------------------------------------------------------
 
 struct gct_batch {
+    static constexpr uint32_t kSideStreams = 4;
     int device = 0;
     std::vector<gct_plan *> plans;    // creation order
-    std::vector<uint32_t> order;      // work order: largest scans first
-    std::vector<BatchScan> scans;
-    uint32_t total_tiles = 0;
+    std::vector<uint32_t> order;      // work order: by kernel group, largest scans first
+    std::vector<BatchScan> scans;     // in work order
+    std::vector<BatchGroup> groups;   // contiguous ranges of `scans`
     DevBuf d_scans, d_counter;
+    // Groups run as concurrent branches: forked from the launching stream to
+    // side streams by events and joined back the same way (also under capture).
+    hipStream_t side[kSideStreams] = {};
+    hipEvent_t fork = nullptr, join[kSideStreams] = {};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<const float *> g_proj;
@@ -746,6 +752,11 @@ struct gct_batch {
     {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        for (uint32_t i = 0; i < kSideStreams; ++i) {
+            if (side[i]) (void)hipStreamDestroy(side[i]);
+            if (join[i]) (void)hipEventDestroy(join[i]);
+        }
+        if (fork) (void)hipEventDestroy(fork);
     }
 };
 
@@ -768,9 +779,29 @@ extern "C" gct_status gct_batch_create(gct_plan *const *plans, uint32_t n, gct_b
             const gct_plan *p = b->plans[i];
             return uint64_t(p->img.x.num_samples) * p->img.y.num_samples * (p->img.dim == 3 ? p->img.z.num_samples : 1u) * p->V;
         };
-        std::stable_sort(b->order.begin(), b->order.end(), [&](uint32_t a, uint32_t c) { return cost(a) > cost(c); });
+        auto key = [&](uint32_t i) {
+            const gct_plan *p = b->plans[i];
+            return std::make_tuple(p->geom.kind, int(p->opts.storage), batch_tile_depth(p->geom.kind, p->circular));
+        };
+        std::stable_sort(b->order.begin(), b->order.end(), [&](uint32_t a, uint32_t c) {
+            return key(a) != key(c) ? key(a) < key(c) : cost(a) > cost(c);
+        });
+        for (uint32_t k = 0; k < n; ++k) {
+            const gct_plan &p = *b->plans[b->order[k]];
+            if (k == 0 || key(b->order[k]) != key(b->order[k - 1])) {
+                const uint32_t depth = batch_tile_depth(p.geom.kind, p.circular);
+                b->groups.push_back(BatchGroup{p.geom.kind, int(p.opts.storage),
+                                               !is_1d(p.geom.kind) && depth == kBatchLdsZU, k, 0, 0});
+            }
+            ++b->groups.back().num_scans;
+        }
         DeviceGuard dg(b->device);
-        b->d_counter = DevBuf(sizeof(uint32_t));
+        b->d_counter = DevBuf(b->groups.size() * sizeof(uint32_t));
+        hip_check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
+        for (uint32_t i = 0; i < gct_batch::kSideStreams; ++i) {
+            hip_check(hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking), "hipStreamCreate");
+            hip_check(hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming), "hipEventCreate");
+        }
         *out = b.release();
     });
 }
@@ -794,21 +825,27 @@ void batch_prepare(gct_batch &b, const float *const *proj, float *const *vol)
     (void)proj;
     b.scans.clear();
     uint32_t tile = 0;
-    for (uint32_t oi : b.order) {
+    size_t gi = 0;
+    for (uint32_t k = 0; k < b.order.size(); ++k) {
+        const uint32_t oi = b.order[k];
+        if (gi < b.groups.size() && b.groups[gi].first_scan == k) tile = 0;   // counts restart per group
         gct_plan &p = *b.plans[oi];
         if (!vol[oi]) fail(GCT_ERR_INVALID_ARGUMENT, "null volume");
         BatchScan sc{};
         sc.bp = bp_args(p, vol[oi], 0, p.V, false, true, 0);
-        sc.bp.z_unroll = 8;
-        sc.bp.variant = GCT_BP_DIRECT;
+        const uint32_t depth = batch_tile_depth(p.geom.kind, sc.bp.circular);
+        sc.bp.z_unroll = depth;
+        sc.bp.variant = depth == kBatchLdsZU ? GCT_BP_LDS : GCT_BP_DIRECT;
         sc.tiles_x = (p.img.x.num_samples + 15u) / 16u;
         sc.tiles_y = (p.img.y.num_samples + 15u) / 16u;
-        sc.tiles_z = is_1d(p.geom.kind) ? 1u : (p.img.z.num_samples + 7u) / 8u;
+        sc.tiles_z = is_1d(p.geom.kind) ? 1u : (p.img.z.num_samples + depth - 1u) / depth;
         sc.tile_begin = tile;
         tile += sc.tiles_x * sc.tiles_y * sc.tiles_z;
         b.scans.push_back(sc);
+        BatchGroup &g = b.groups[gi];
+        g.total_tiles = tile;
+        if (k + 1 == g.first_scan + g.num_scans) ++gi;
     }
-    b.total_tiles = tile;
     const size_t bytes = b.scans.size() * sizeof(BatchScan);
     if (b.d_scans.bytes != bytes) b.d_scans = DevBuf(bytes);
     hip_check(hipMemcpy(b.d_scans.p, b.scans.data(), bytes, hipMemcpyHostToDevice), "upload batch table");
@@ -822,8 +859,22 @@ void batch_record(gct_batch &b, const float *const *proj, hipStream_t s)
         if (!proj[oi]) fail(GCT_ERR_INVALID_ARGUMENT, "null projections");
         launch_filter(filter_args(p, proj[oi], 0, 0, p.V, 0), s);
     }
-    launch_backproject_batch(b.d_scans.as<BatchScan>(), uint32_t(b.scans.size()), b.total_tiles,
-                             b.d_counter.as<uint32_t>(), s);
+    uint32_t *counters = b.d_counter.as<uint32_t>();
+    launch_zero_counters(counters, uint32_t(b.groups.size()), s);
+    if (b.groups.size() == 1) {
+        launch_backproject_batch_group(b.d_scans.as<BatchScan>(), b.groups[0], counters, s);
+        return;
+    }
+    hip_check(hipEventRecord(b.fork, s), "fork");
+    const uint32_t used = std::min<uint32_t>(gct_batch::kSideStreams, uint32_t(b.groups.size()));
+    for (uint32_t i = 0; i < used; ++i) hip_check(hipStreamWaitEvent(b.side[i], b.fork, 0), "fork wait");
+    for (size_t gi = 0; gi < b.groups.size(); ++gi)
+        launch_backproject_batch_group(b.d_scans.as<BatchScan>(), b.groups[gi], counters + gi,
+                                       b.side[gi % gct_batch::kSideStreams]);
+    for (uint32_t i = 0; i < used; ++i) {
+        hip_check(hipEventRecord(b.join[i], b.side[i]), "join");
+        hip_check(hipStreamWaitEvent(s, b.join[i], 0), "join wait");
+    }
 }
 
 }  // namespace

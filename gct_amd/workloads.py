@@ -14,7 +14,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import (CONE_PARALLEL, CYLINDRICAL, FAN_ARC_1D, FAN_LINE_1D, FLAT, PARALLEL_1D, PARALLEL_2D, Image, Scan,
-               WINDOW_NONE, WINDOW_SHEPP_LOGAN, circular_scan, is_1d, phantom_project, phantom_random)
+               WINDOW_NONE, WINDOW_SHEPP_LOGAN, circular_scan, is_1d, phantom_project, phantom_project_device,
+               phantom_random)
 
 
 @dataclass
@@ -31,6 +32,14 @@ class Workload:
 
     def projections(self, nthreads: int = 0) -> np.ndarray:
         return phantom_project(self.scan, self.phantom, self.i0, self.seed + 7, nthreads)
+
+    def projections_device(self, out=None, stream=None):
+        """The same projections made by the GPU generator into a device
+        tensor (allocated on the current device when `out` is None)."""
+        import torch
+        if out is None:
+            out = torch.empty(self.scan.proj_shape, dtype=torch.float32, device="cuda")
+        return phantom_project_device(self.scan, self.phantom, out, self.i0, self.seed + 7, stream)
 
     @property
     def updates(self) -> int:
@@ -105,6 +114,55 @@ def uneven_batch(num_slices: int = 64, num_slabs: int = 16, seed: int = 1004) ->
     out = [fan_arc_slice(rng, i) for i in range(num_slices)]
     out += [cylindrical_slab(i) for i in range(num_slabs)]
     return out
+
+
+def serving_request(rng: np.random.Generator, idx: int) -> Workload:
+    """One request of the serving stream, drawn from the mix:
+    40 % parallel / cone-parallel volumes of 128^3..256^3 (1 mm voxels,
+    360..720 views over 180 deg), 40 % flat cone beam of 256^3..512^3
+    (360..720 views, full 360 deg or short 180 deg + fan scans, detector
+    512..1024 x 384..768 at 0.4 mm, SOD 500 mm, SDD 1000 mm, the volume sized
+    to the detector's field of view), 20 % fan-arc slices as in the uneven
+    batch.  Phantoms: 12 random ellipsoids (10 ellipses for slices)."""
+    u = rng.random()
+    seed = 7000 + idx
+    if u < 0.4:
+        kind = PARALLEL_2D if rng.random() < 0.5 else CONE_PARALLEL
+        n = int(rng.integers(128, 257))
+        views = int(rng.integers(360, 721))
+        nu = int(math.ceil(n * math.sqrt(2.0))) + 4
+        if kind == PARALLEL_2D:
+            scan = circular_scan(kind, nu, 1.0, views, math.pi / views, nv=n + 4, dv=1.0,
+                                 det_row_center=(nu - 1) / 2.0, det_col_center=(n + 3) / 2.0)
+        else:
+            sid, sdd = 500.0, 1000.0
+            nv = int(math.ceil(n * 1.1)) + 4
+            scan = circular_scan(kind, nu, 1.0, views, math.pi / views, nv=nv, dv=sdd / sid, det_row_center=(nu - 1) / 2.0,
+                                 det_col_center=(nv - 1) / 2.0, src_det_dist=sdd, src_iso_dist=sid)
+        img = Image(n, n, 1.0, 1.0, n, 1.0)
+        ph = phantom_random(seed, 12, 3, 0.5 * n, (0.05, 0.35), (-0.02, 0.1))
+        return Workload(f"serve_{idx}_{'par2d' if kind == PARALLEL_2D else 'conepar'}_{n}", scan, img, ph, 0.0, seed)
+    if u < 0.8:
+        n = int(rng.integers(256, 513))
+        views = int(rng.integers(360, 721))
+        nu = int(rng.integers(512, 1025))
+        nv = int(rng.integers(384, 769))
+        sid, sdd, pitch = 500.0, 1000.0, 0.4
+        short = rng.random() < 0.5
+        half_fan = math.atan(0.5 * nu * pitch / sdd)
+        span = (math.pi + 2.0 * half_fan) if short else 2.0 * math.pi
+        scan = circular_scan(FLAT, nu, pitch, views, span / views, nv=nv, dv=pitch, det_row_center=(nu - 1) / 2.0,
+                             det_col_center=(nv - 1) / 2.0, src_det_dist=sdd, src_iso_dist=sid)
+        vox = 0.95 * pitch * (sid / sdd) * min(nu, nv) / n
+        img = Image(n, n, vox, vox, n, vox)
+        ph = phantom_random(seed, 12, 3, 0.5 * n * vox, (0.05, 0.35), (-0.02, 0.1))
+        return Workload(f"serve_{idx}_flat{'_short' if short else ''}_{n}", scan, img, ph, 0.0, seed)
+    return fan_arc_slice(rng, idx)
+
+
+def serving_mix(num_requests: int = 32, seed: int = 1005) -> list[Workload]:
+    rng = np.random.default_rng(seed)
+    return [serving_request(rng, i) for i in range(num_requests)]
 
 
 def small_case(kind: int, n: int = 64, views: int = 120, nu: int = 96, nv: int = 80, short_scan: bool = False,

@@ -336,6 +336,14 @@ gct_status gct_phantom_project(const gct_ct_geom *geom, int dim, const double *p
                                double i0, uint64_t noise_seed, float *proj, int nthreads);
 /* Phantom rasterised at voxel centres into img (used by reprojection tests). */
 gct_status gct_phantom_rasterize(const gct_image_geom *img, const double *params, uint32_t count, float *vol);
+/* Device twins of the two calls above: the same FP64 arithmetic (and the same
+ * per-row noise streams) on the current device, written to device memory and
+ * enqueued on `stream` (hipStream_t, NULL = null stream); they return when the
+ * work is done.  At most 64 shapes. */
+gct_status gct_phantom_project_device(const gct_ct_geom *geom, int dim, const double *params, uint32_t count,
+                                      double i0, uint64_t noise_seed, float *proj_dev, void *stream);
+gct_status gct_phantom_rasterize_device(const gct_image_geom *img, const double *params, uint32_t count,
+                                        float *vol_dev, void *stream);
 
 #ifdef __cplusplus
 }
