@@ -149,6 +149,7 @@ struct FpLaunch {
     float du, dv;
     const gct_proj_geom *views;   // device copy
     const float *vol;
+    float *vol_t;                 // scratch for the x/y-transposed volume ([z][x][y]); null: read `vol` only
     float *proj;
 };
 

@@ -417,7 +417,7 @@ __device__ inline void bp_tile3d(const BpLaunch &a, uint32_t bx, uint32_t by, ui
                 float w = s.w;
                 if constexpr (KIND == GCT_CYLINDRICAL) {
                     const float nvk = fmaf(zk, s.dnv, s.nv0);
-                    w = s.w / fmaf(nvk, nvk, s.rho2);
+                    w = s.w * __builtin_amdgcn_rcpf(fmaf(nvk, nvk, s.rho2));
                 }
                 acc[k] = fmaf(val, w, acc[k]);
             }
@@ -427,7 +427,7 @@ __device__ inline void bp_tile3d(const BpLaunch &a, uint32_t bx, uint32_t by, ui
                 const ColSetup s = setup_column<KIND, F64>(a, v, fx, fy, fz + static_cast<float>(k), nuf);
                 const float val = bilinear<S>(p, a.pitch, col_sample(s.col, nuf), s.row0, nvf);
                 float w = s.w;
-                if constexpr (KIND == GCT_CYLINDRICAL) w = s.w / fmaf(s.nv0, s.nv0, s.rho2);
+                if constexpr (KIND == GCT_CYLINDRICAL) w = s.w * __builtin_amdgcn_rcpf(fmaf(s.nv0, s.nv0, s.rho2));
                 acc[k] = fmaf(val, w, acc[k]);
             }
         }
@@ -639,7 +639,7 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                         float w = s.w;
                         if constexpr (KIND == GCT_CYLINDRICAL) {
                             const float nvk = fmaf(zk, s.dnv, s.nv0);
-                            w = s.w / fmaf(nvk, nvk, s.rho2);
+                            w = s.w * __builtin_amdgcn_rcpf(fmaf(nvk, nvk, s.rho2));
                         }
                         acc[k] = fmaf(val, w, acc[k]);
                     }
@@ -654,7 +654,7 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                         float w = s.w;
                         if constexpr (KIND == GCT_CYLINDRICAL) {
                             const float nvk = fmaf(zk, s.dnv, s.nv0);
-                            w = s.w / fmaf(nvk, nvk, s.rho2);
+                            w = s.w * __builtin_amdgcn_rcpf(fmaf(nvk, nvk, s.rho2));
                         }
                         acc[k] = fmaf(val, w, acc[k]);
                     }
@@ -828,6 +828,11 @@ __global__ void unpack_kernel(const void *src, float *dst, uint32_t nu, uint32_t
 // the planes of voxel centres normal to the axis the ray is most aligned with
 // (in units of voxels), interpolate bilinearly in each plane (zero outside),
 // and scale by the path length per plane.  Rays from a source count t > 0.
+// Sum over the planes of voxel centres normal to the ray's dominant axis of
+// the bilinear sample in each plane, times the step length.  Plane i is
+// crossed at t(i) = t0 + i dt; the two in-plane coordinates (in voxel units)
+// are linear in i, so after an FP64 set-up per ray each plane costs two FMAs,
+// two floors and four predicated taps (zero outside the volume).
 __device__ float joseph_ray(const FpLaunch &a, const float o[3], const float d[3], bool from_source)
 {
     const bool is3 = !is_1d(a.kind);
@@ -859,47 +864,86 @@ __device__ float joseph_ray(const FpLaunch &a, const float o[3], const float d[3
     if (!(thi > tlo)) return 0.0f;
     const float pa0 = o[ax] + tlo * d[ax], pa1 = o[ax] + thi * d[ax];
     const float ia = (fminf(pa0, pa1) - first[ax]) / sp[ax], ib = (fmaxf(pa0, pa1) - first[ax]) / sp[ax];
-    const int i0 = max(0, static_cast<int>(floorf(ia))), i1 = min(n[ax] - 1, static_cast<int>(ceilf(ib)));
+    int i0 = max(0, static_cast<int>(floorf(ia))), i1 = min(n[ax] - 1, static_cast<int>(ceilf(ib)));
+    const int oa = (ax == 0) ? 1 : 0, ob = (ax == 2) ? 1 : 2;
+
+    const double dax = d[ax];
+    const double t0 = (static_cast<double>(first[ax]) - o[ax]) / dax, dt = static_cast<double>(sp[ax]) / dax;
+    if (from_source) {   // only planes in front of the source: t(i) > 0
+        const double ic = -t0 / dt;
+        if (dt > 0.0) i0 = max(i0, static_cast<int>(floor(ic)) + 1);
+        else i1 = min(i1, static_cast<int>(ceil(ic)) - 1);
+    }
+    if (i1 < i0) return 0.0f;
+    const float fa0 = static_cast<float>((o[oa] + t0 * d[oa] - first[oa]) / sp[oa]);
+    const float dfa = static_cast<float>(dt * d[oa] / sp[oa]);
     const float len = sqrtf(d[0] * d[0] + d[1] * d[1] + (is3 ? d[2] * d[2] : 0.0f));
     const float step = sp[ax] * len / fabsf(d[ax]);
-    int oa = (ax == 0) ? 1 : 0, ob = (ax == 2) ? 1 : 2;
-    const size_t sx = 1, sy = a.nx, sz = static_cast<size_t>(a.nx) * a.ny;
-    const size_t stride[3] = {sx, sy, sz};
+    // Rays marching along x read the x/y-transposed copy, so that the lanes of
+    // a wave (neighbouring detector columns) always spread along the
+    // contiguous axis of the copy they read.
+    const bool tr = ax == 0 && a.vol_t != nullptr;
+    const uint32_t stride[3] = {tr ? a.ny : 1u, tr ? 1u : a.nx, a.nx * a.ny};
+    const uint32_t na = static_cast<uint32_t>(n[oa]), sa = stride[oa];
+    const float *plane = (tr ? a.vol_t : a.vol) + static_cast<size_t>(i0) * stride[ax];
+    const size_t pstep = stride[ax];
     float acc = 0.0f;
-    for (int i = i0; i <= i1; ++i) {
-        const float pa = first[ax] + static_cast<float>(i) * sp[ax];
-        const float t = (pa - o[ax]) / d[ax];
-        if (from_source && !(t > 0.0f)) continue;
-        const float fa = (o[oa] + t * d[oa] - first[oa]) / sp[oa];
-        const float ffa = floorf(fa);
-        const int a0 = static_cast<int>(ffa);
-        const float wa = fa - ffa;
-        if (!is3) {
-            float s = 0.0f;
-            if (a0 >= 0 && a0 < n[oa]) s += (1.0f - wa) * a.vol[static_cast<size_t>(i) * stride[ax] + static_cast<size_t>(a0) * stride[oa]];
-            if (a0 + 1 >= 0 && a0 + 1 < n[oa]) s += wa * a.vol[static_cast<size_t>(i) * stride[ax] + static_cast<size_t>(a0 + 1) * stride[oa]];
-            acc += s;
-        } else {
-            const float fb = (o[ob] + t * d[ob] - first[ob]) / sp[ob];
-            const float ffb = floorf(fb);
-            const int b0 = static_cast<int>(ffb);
-            const float wb = fb - ffb;
-            float s = 0.0f;
-            for (int da = 0; da < 2; ++da) {
-                const int ia2 = a0 + da;
-                if (ia2 < 0 || ia2 >= n[oa]) continue;
-                for (int db = 0; db < 2; ++db) {
-                    const int ib2 = b0 + db;
-                    if (ib2 < 0 || ib2 >= n[ob]) continue;
-                    const float w = (da ? wa : 1.0f - wa) * (db ? wb : 1.0f - wb);
-                    s += w * a.vol[static_cast<size_t>(i) * stride[ax] + static_cast<size_t>(ia2) * stride[oa] +
-                                   static_cast<size_t>(ib2) * stride[ob]];
-                }
-            }
-            acc += s;
+    if (!is3) {
+        for (int i = i0; i <= i1; ++i, plane += pstep) {
+            const float fa = fmaf(static_cast<float>(i), dfa, fa0);
+            const float ffa = floorf(fa);
+            const int a0 = static_cast<int>(ffa);
+            const float wa = fa - ffa;
+            const float v0 = static_cast<uint32_t>(a0) < na ? plane[static_cast<uint32_t>(a0) * sa] : 0.0f;
+            const float v1 = static_cast<uint32_t>(a0 + 1) < na ? plane[static_cast<uint32_t>(a0 + 1) * sa] : 0.0f;
+            acc += fmaf(wa, v1 - v0, v0);
         }
+        return acc * step;
+    }
+    const float fb0 = static_cast<float>((o[ob] + t0 * d[ob] - first[ob]) / sp[ob]);
+    const float dfb = static_cast<float>(dt * d[ob] / sp[ob]);
+    const uint32_t nb = static_cast<uint32_t>(n[ob]), sb = stride[ob];
+#pragma unroll 2
+    for (int i = i0; i <= i1; ++i, plane += pstep) {
+        const float fi = static_cast<float>(i);
+        const float fa = fmaf(fi, dfa, fa0), fb = fmaf(fi, dfb, fb0);
+        const float ffa = floorf(fa), ffb = floorf(fb);
+        const int a0 = static_cast<int>(ffa), b0 = static_cast<int>(ffb);
+        const float wa = fa - ffa, wb = fb - ffb;
+        const bool ia0 = static_cast<uint32_t>(a0) < na, ia1 = static_cast<uint32_t>(a0 + 1) < na;
+        const bool ib0 = static_cast<uint32_t>(b0) < nb, ib1 = static_cast<uint32_t>(b0 + 1) < nb;
+        const uint32_t o00 = static_cast<uint32_t>(a0) * sa + static_cast<uint32_t>(b0) * sb;
+        const float v00 = (ia0 && ib0) ? plane[o00] : 0.0f;
+        const float v10 = (ia1 && ib0) ? plane[o00 + sa] : 0.0f;
+        const float v01 = (ia0 && ib1) ? plane[o00 + sb] : 0.0f;
+        const float v11 = (ia1 && ib1) ? plane[o00 + sa + sb] : 0.0f;
+        const float lo = fmaf(wa, v10 - v00, v00), hi = fmaf(wa, v11 - v01, v01);
+        acc += fmaf(wb, hi - lo, lo);
     }
     return acc * step;
+}
+
+// [z][y][x] -> [z][x][y], 32 x 32 tiles through LDS (both sides coalesced).
+__global__ __launch_bounds__(256) void transpose_xy_kernel(const float *__restrict__ src, float *__restrict__ dst,
+                                                           uint32_t nx, uint32_t ny, uint32_t tiles_x, uint32_t tiles_y)
+{
+    __shared__ float t[32][33];
+    const uint32_t b = blockIdx.x;
+    const uint32_t tx = b % tiles_x, rest = b / tiles_x;
+    const uint32_t ty = rest % tiles_y, z = rest / tiles_y;
+    const size_t slice = static_cast<size_t>(nx) * ny;
+    const float *sp = src + z * slice;
+    float *dp = dst + z * slice;
+    const uint32_t lx = threadIdx.x & 31u, ly = threadIdx.x >> 5;
+    for (uint32_t k = 0; k < 32u; k += 8u) {
+        const uint32_t x = tx * 32u + lx, y = ty * 32u + ly + k;
+        if (x < nx && y < ny) t[ly + k][lx] = sp[static_cast<size_t>(y) * nx + x];
+    }
+    __syncthreads();
+    for (uint32_t k = 0; k < 32u; k += 8u) {
+        const uint32_t y = ty * 32u + lx, x = tx * 32u + ly + k;
+        if (x < nx && y < ny) dp[static_cast<size_t>(x) * ny + y] = t[lx][ly + k];
+    }
 }
 
 __global__ __launch_bounds__(256) void joseph_kernel(FpLaunch a)
@@ -1150,6 +1194,11 @@ void launch_forward_project(const FpLaunch &a, hipStream_t s)
 {
     const size_t total = static_cast<size_t>(a.nviews) * (is_1d(a.kind) ? 1u : a.nv) * a.nu;
     if (total == 0) return;
+    if (a.vol_t) {
+        const uint32_t tx = (a.nx + 31u) / 32u, ty = (a.ny + 31u) / 32u;
+        hipLaunchKernelGGL(transpose_xy_kernel, dim3(tx * ty * a.nz), dim3(256), 0, s, a.vol, a.vol_t, a.nx, a.ny, tx,
+                           ty);
+    }
     hipLaunchKernelGGL(joseph_kernel, dim3(static_cast<uint32_t>((total + 255) / 256)), dim3(256), 0, s, a);
     hip_check(hipGetLastError(), "forward projection");
 }

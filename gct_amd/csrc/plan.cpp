@@ -388,13 +388,15 @@ void check_range(const gct_plan &p, uint32_t first, uint32_t n)
 
 }  // namespace
 
-FpLaunch fp_args(gct_plan &p, const float *vol, float *proj)
+FpLaunch fp_args(gct_plan &p, const float *vol, float *proj, float *vol_t)
 {
     FpLaunch a{};
     a.kind = p.geom.kind;
     a.nx = p.img.x.num_samples;
     a.ny = p.img.y.num_samples;
     a.nz = (p.img.dim == 3) ? p.img.z.num_samples : 1u;
+    if (uint64_t(a.nx) * a.ny * a.nz >= (uint64_t(1) << 32))
+        fail(GCT_ERR_UNSUPPORTED, "forward projection needs a volume below 2^32 voxels (32-bit in-plane offsets)");
     a.dx = p.img.x.sampling_width;
     a.dy = p.img.y.sampling_width;
     a.dz = (p.img.dim == 3) ? p.img.z.sampling_width : 1.0f;
@@ -408,6 +410,7 @@ FpLaunch fp_args(gct_plan &p, const float *vol, float *proj)
     a.dv = is_1d(p.geom.kind) ? 1.0f : p.geom.det_col.sampling_width;
     a.views = p.d_views.as<gct_proj_geom>();
     a.vol = vol;
+    a.vol_t = vol_t;
     a.proj = proj;
     return a;
 }
@@ -581,7 +584,9 @@ extern "C" gct_status gct_plan_forward_project(gct_plan *p, const float *vol, fl
     return guarded([&] {
         if (!p || !vol || !proj) fail(GCT_ERR_INVALID_ARGUMENT, "null argument");
         DeviceGuard dg(p->device);
-        const FpLaunch a = fp_args(*p, vol, proj);
+        // the transposed-volume scratch is allocated on first use and kept with the plan
+        if (!p->d_vol_t.p) p->d_vol_t = DevBuf(volume_bytes(p->img));
+        const FpLaunch a = fp_args(*p, vol, proj, p->d_vol_t.as<float>());
         launch_forward_project(a, static_cast<hipStream_t>(stream));
     });
 }

@@ -57,7 +57,7 @@ uint64_t request_footprint(const gct_ct_geom &g, const gct_image_geom &img, cons
     const uint64_t nv = is_1d(g.kind) ? 1u : g.det_col.num_samples;
     const uint64_t proj = uint64_t(g.num_proj) * nv * g.det_row.num_samples * sizeof(float);
     uint64_t b = proj + volume_bytes(img) + plan_device_bytes(g, img, o, 0);
-    if (reproject) b += proj + 2u * kResidualBlocks * sizeof(double);
+    if (reproject) b += proj + volume_bytes(img) + 2u * kResidualBlocks * sizeof(double);
     return b;
 }
 
@@ -101,11 +101,12 @@ void gct_server::process(Request &r, hipStream_t s)
         launch_filter(filter_args(*plan, dproj.as<float>(), 0, 0, plan->V, 0), s);
         launch_backproject(bp_args(*plan, dvol.as<float>(), 0, plan->V, false, true, 0), s);
         std::vector<double> part;
-        DevBuf dre, dpart;
+        DevBuf dre, dvt, dpart;
         if (r.reproject) {
             dre = DevBuf(npix * sizeof(float), s, true);
+            dvt = DevBuf(volume_bytes(r.img), s, true);
             dpart = DevBuf(2u * kResidualBlocks * sizeof(double), s, true);
-            launch_forward_project(fp_args(*plan, dvol.as<float>(), dre.as<float>()), s);
+            launch_forward_project(fp_args(*plan, dvol.as<float>(), dre.as<float>(), dvt.as<float>()), s);
             launch_residual(dre.as<float>(), dproj.as<float>(), npix, dpart.as<double>(), kResidualBlocks, s);
             part.resize(2u * kResidualBlocks);
             hip_check(hipMemcpyAsync(part.data(), dpart.p, dpart.bytes, hipMemcpyDeviceToHost, s), "download residual");
