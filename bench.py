@@ -11,7 +11,11 @@ flat      The headline number.  A step is one plain FDK reconstruction of the
           backprojection, projections already resident in HBM.  The plan is
           autotuned once before the warm-up.  `value` is voxel x view updates
           per second over the timed steps; the backprojection kernel's
-          roofline is measured with HIP events on its stream.
+          roofline is measured with HIP events on its stream.  Beside it,
+          the same backprojection launched one view at a time with a
+          full-volume read-modify-write per view (the reference's structure).
+precision The short-scan C-arm workload (512^3, 248 views over 200 deg) with
+          FP32, FP16 and BF16 filtered-projection storage.
 batched   The uneven batch: 64 fan-arc slices of seeded sizes plus 16
           cylindrical 512 x 512 x 64 slabs (672 x 64 detector, 1152 views).
           Plain = each scan's FDK in turn, waiting for each (one scan per
@@ -25,7 +29,7 @@ served    The serving stream: 32 seeded requests of mixed geometry and size,
           requests queued at once: capacity), then with Poisson arrivals at
           80 % of that capacity (latency).
 
-    python bench.py [--steps K] [--warmup W] [--no-cpu-baseline] [--sections flat,batched,served]
+    python bench.py [--steps K] [--warmup W] [--no-cpu-baseline] [--sections flat,precision,batched,served]
 """
 from __future__ import annotations
 
@@ -142,6 +146,20 @@ def run_flat(a, g, W, torch, np):
         except (OSError, ValueError):
             traffic = None
 
+    # The reference's structure on the same kernels: one view per launch, each
+    # launch reading and writing the whole volume (fdk_axial.cuh:189-199).
+    V = wl.scan.num_proj
+    per_view = []
+    for rep in range(2):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for i in range(V):
+            plan.backproject(v, first=i, n=1, accumulate=i > 0, finish=i == V - 1, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        per_view.append(e0.elapsed_time(e1))
+    per_view_ms = min(per_view)
+
     vol = v.cpu().numpy()
     truth = g.phantom_rasterize(wl.img, wl.phantom)
     rmse_truth = float(np.sqrt(np.mean((vol - truth) ** 2)))
@@ -174,6 +192,12 @@ def run_flat(a, g, W, torch, np):
         "bp_ms": round(bp_mean, 4),
         "latency_ms": {"p50": round(_percentile(step_ms, 0.5), 4), "p99": round(_percentile(step_ms, 0.99), 4)},
         "rmse_vs_phantom": rmse_truth,
+        "reference_structured_bp": {
+            "what": "same kernels launched one view at a time, full-volume read-modify-write per view",
+            "bp_ms": round(per_view_ms, 3),
+            "bp_gups": round(upd / (per_view_ms * 1e-3) / 1e9, 2),
+            "fused_vs_per_view": round(per_view_ms / bp_mean, 3),
+        },
         "roofline": {
             "bound": "valu",
             "kernel": kernel or f"bp3d_{variant}_kernel<FLAT, f32, ZU={info.z_unroll}>",
@@ -191,6 +215,52 @@ def run_flat(a, g, W, torch, np):
     del p, v
     plan.close()
     return out, wl, proj, vol
+
+
+# ---- precision forms on the C-arm short scan ----------------------------------------
+
+def run_precision(a, g, W, torch, np):
+    """FP32 / FP16 / BF16 filtered-projection storage (FP32 accumulation, FP64
+    per-view tables and column set-up at 512^3) on the short-scan C-arm
+    workload: throughput and error against the rasterised phantom and the
+    FP32 form.  (Errors against the FP64 oracle: tests/test_gpu_precision.py.)"""
+    wl = W.carm_short_scan_512()
+    stream = torch.cuda.current_stream()
+    p = wl.projections_device()
+    truth = torch.empty(wl.img.shape, dtype=torch.float32, device="cuda")
+    g.phantom_rasterize_device(wl.img, wl.phantom, truth)
+    forms = {}
+    ref = None
+    for name, st in (("f32", g.STORE_F32), ("f16", g.STORE_F16), ("bf16", g.STORE_BF16)):
+        plan = g.Plan(wl.scan, wl.img, storage=st)
+        v = torch.empty(wl.img.shape, dtype=torch.float32, device="cuda")
+        plan.execute(p, v, stream=stream)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(3):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            plan.execute(p, v, stream=stream)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        ms = statistics.median(ts)
+        info = plan.info()
+        d = v - truth
+        f = {"ms": round(ms, 3), "gups": round(wl.updates / (ms * 1e-3) / 1e9, 2),
+             "filtered_bytes": int(info.filt_bytes), "setup_f64": bool(info.setup_f64),
+             "rmse_vs_phantom": float(torch.sqrt(torch.mean(d * d)))}
+        if ref is None:
+            ref = v.clone()
+        else:
+            e = v - ref
+            f["rmse_vs_f32"] = float(torch.sqrt(torch.mean(e * e)))
+            f["max_abs_vs_f32"] = float(e.abs().max())
+        forms[name] = f
+        plan.close()
+        del v
+    return {"workload": "carm_short_scan_512: SOD 750, SDD 1200 mm, 248 views over 200 deg (Parker), 616x480 det "
+                        "@0.616 mm, 512^3 @0.3 mm, Poisson I0=1e5", "updates": wl.updates, "forms": forms}
 
 
 # ---- uneven batch: plain per-scan calls vs one graph ------------------------------
@@ -357,7 +427,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--sections", default="flat,batched,served")
+    ap.add_argument("--sections", default="flat,precision,batched,served")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy 5 times (reference kernel for HBM byte counters)")
@@ -387,6 +457,10 @@ def main():
         return
 
     out, wl, proj, vol = run_flat(a, g, W, torch, np)
+    if "precision" in sections:
+        print("bench: precision section", file=sys.stderr, flush=True)
+        out["precision"] = run_precision(a, g, W, torch, np)
+        torch.cuda.empty_cache()
     if "batched" in sections:
         print("bench: batched section", file=sys.stderr, flush=True)
         out["batched"] = run_batched(a, g, W, torch, np)

@@ -1,0 +1,693 @@
+// Device templates of the backprojection kernels (voxel-driven, direct and
+// LDS-staged; persistent batch kernels) and their launch helpers.  Included
+// by the per-geometry translation units bp_1d.hip, bp_parallel.hip,
+// bp_cyl.hip and bp_flat.hip, each of which instantiates only its own
+// geometry kinds, and by kernels.hip for the storage types.
+//
+// Reference stages replaced: backProjKern / backProjVoxDriven
+// (back_proj.cuh:16-104) with its per-view texture copies
+// (copy_host_dev.cuh:99-147) and the final setVal / operator*= on the volume
+// (nd_array_dev.cuh:58-99).
+#pragma once
+
+#include "common.hpp"
+
+#include <algorithm>
+#include <cstdint>
+
+namespace gct {
+namespace {
+
+// ---- storage of filtered projections -------------------------------------
+
+template <int S> struct Stor;
+template <> struct Stor<GCT_STORE_F32> {
+    using T = float;
+    __device__ static float ld(const T *p, uint32_t i) { return p[i]; }
+    __device__ static T cv(float x) { return x; }
+};
+template <> struct Stor<GCT_STORE_F16> {
+    using T = _Float16;
+    __device__ static float ld(const T *p, uint32_t i) { return static_cast<float>(p[i]); }
+    __device__ static T cv(float x) { return static_cast<_Float16>(x); }
+};
+template <> struct Stor<GCT_STORE_BF16> {
+    using T = uint16_t;
+    __device__ static float ld(const T *p, uint32_t i) { return __uint_as_float(uint32_t(p[i]) << 16); }
+    __device__ static T cv(float x)
+    {
+        uint32_t u = __float_as_uint(x);
+        u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even
+        return static_cast<uint16_t>(u >> 16);
+    }
+};
+
+// ---- backprojection -------------------------------------------------------
+
+struct Sample {
+    uint32_t colbase;   // padded column index of the left tap
+    float alpha;        // horizontal fraction
+};
+
+// Clamping the coordinate to [-1, n] keeps every tap inside the padded
+// buffer; any coordinate outside the detector then lands on zero padding,
+// which reproduces the border-zero texture (nd_array_texture.cuh:43-50).
+__device__ inline Sample col_sample(float col, float nu)
+{
+    const float c = __builtin_amdgcn_fmed3f(col, -1.0f, nu);
+    const float f = floorf(c);
+    return Sample{static_cast<uint32_t>(static_cast<int>(f) + 1), c - f};
+}
+
+template <int S>
+__device__ inline float bilinear(const typename Stor<S>::T *__restrict__ p, uint32_t pitch, Sample cs, float row,
+                                 float nvf)
+{
+    const float r = __builtin_amdgcn_fmed3f(row, -1.0f, nvf);
+    const float fr = floorf(r);
+    const float beta = r - fr;
+    const uint32_t off = static_cast<uint32_t>(static_cast<int>(fr) + 1) * pitch + cs.colbase;
+    const float t0 = Stor<S>::ld(p, off), t1 = Stor<S>::ld(p, off + 1u);
+    const float b0 = Stor<S>::ld(p, off + pitch), b1 = Stor<S>::ld(p, off + pitch + 1u);
+    const float top = fmaf(cs.alpha, t1 - t0, t0);
+    const float bot = fmaf(cs.alpha, b1 - b0, b0);
+    return fmaf(beta, bot - top, top);
+}
+
+template <int S>
+__device__ inline float linear(const typename Stor<S>::T *__restrict__ p, uint32_t pitch, Sample cs)
+{
+    const uint32_t off = pitch + cs.colbase;
+    const float t0 = Stor<S>::ld(p, off), t1 = Stor<S>::ld(p, off + 1u);
+    return fmaf(cs.alpha, t1 - t0, t0);
+}
+
+// Per (thread, view) setup of a voxel column, shared by all z voxels.
+struct ColSetup {
+    float col;     // column coordinate (unclamped)
+    float w;       // weight (constant along z unless cylindrical)
+    float row0;    // row coordinate of the first z voxel
+    float drow;    // row step per z voxel
+    float rho2;    // cylindrical: in-plane squared distance
+    float nv0;     // cylindrical: v numerator of the first z voxel
+    float dnv;     // cylindrical: v numerator step
+};
+
+template <int KIND, bool F64>
+__device__ inline ColSetup setup_column(const BpLaunch &a, uint32_t v, float fx, float fy, float fz, float nuf)
+{
+    ColSetup s{};
+    if constexpr (KIND == GCT_PARALLEL_2D || KIND == GCT_CONE_PARALLEL) {
+        // The column is an affine form with a large constant (the detector
+        // centre); evaluating it in FP64 once per column keeps it as accurate
+        // as the reference's FP32 chain t = x cos + y sin, col = t / du + cu.
+        const BpRec64 &r = a.rec64[v];
+        s.col = static_cast<float>(r.u[0] + r.u[1] * static_cast<double>(fx) + r.u[2] * static_cast<double>(fy));
+        s.w = 1.0f;
+        s.row0 = static_cast<float>(r.v[0] + r.v[3] * static_cast<double>(fz));
+        s.drow = static_cast<float>(r.v[3]);
+    } else if constexpr (KIND == GCT_FLAT) {
+        if constexpr (F64) {
+            const BpRec64 &r = a.rec64[v];
+            const double X = fx, Y = fy, Z = fz;
+            const double nw = r.w[0] + r.w[1] * X + r.w[2] * Y + r.w[3] * Z;
+            const double nu = r.u[0] + r.u[1] * X + r.u[2] * Y + r.u[3] * Z;
+            const double nv = r.v[0] + r.v[1] * X + r.v[2] * Y + r.v[3] * Z;
+            const double rd = 1.0 / nw;
+            s.col = static_cast<float>(r.k[0] + nu * rd);
+            s.w = static_cast<float>(rd * rd);
+            s.row0 = static_cast<float>(r.k[1] + nv * rd);
+            s.drow = static_cast<float>(r.v[3] * rd);
+        } else {
+            const BpRec &r = a.rec[v];
+            const float nw = fmaf(r.w[3], fz, fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0])));
+            const float nu = fmaf(r.u[3], fz, fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0])));
+            const float nv = fmaf(r.v[3], fz, fmaf(r.v[2], fy, fmaf(r.v[1], fx, r.v[0])));
+            const float rd = 1.0f / nw;
+            s.col = fmaf(nu, rd, r.k[0]);
+            s.w = rd * rd;
+            s.row0 = fmaf(nv, rd, r.k[1]);
+            s.drow = r.v[3] * rd;
+        }
+    } else if constexpr (KIND == GCT_CYLINDRICAL) {
+        if constexpr (F64) {
+            const BpRec64 &r = a.rec64[v];
+            const double X = fx, Y = fy, Z = fz;
+            const double nw = r.w[0] + r.w[1] * X + r.w[2] * Y + r.w[3] * Z;
+            const double nu = r.u[0] + r.u[1] * X + r.u[2] * Y + r.u[3] * Z;
+            const double nv = r.v[0] + r.v[1] * X + r.v[2] * Y + r.v[3] * Z;
+            const double rho2 = nu * nu + nw * nw;
+            const double rr = 1.0 / sqrt(rho2);
+            s.col = static_cast<float>(r.k[0] + atan2(nu, nw) * static_cast<double>(a.inv_dgamma));
+            s.row0 = static_cast<float>(r.k[1] + r.k[2] * nv * rr);
+            s.drow = static_cast<float>(r.k[2] * r.v[3] * rr);
+            s.rho2 = static_cast<float>(rho2);
+            s.nv0 = static_cast<float>(nv);
+            s.dnv = static_cast<float>(r.v[3]);
+            s.w = static_cast<float>(r.k[3]);
+        } else {
+            const BpRec &r = a.rec[v];
+            const float nw = fmaf(r.w[3], fz, fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0])));
+            const float nu = fmaf(r.u[3], fz, fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0])));
+            const float nv = fmaf(r.v[3], fz, fmaf(r.v[2], fy, fmaf(r.v[1], fx, r.v[0])));
+            const float rho2 = fmaf(nu, nu, nw * nw);
+            const float rr = 1.0f / sqrtf(rho2);
+            s.col = fmaf(atan2f(nu, nw), a.inv_dgamma, r.k[0]);
+            s.row0 = fmaf(r.k[2] * nv, rr, r.k[1]);
+            s.drow = r.k[2] * r.v[3] * rr;
+            s.rho2 = rho2;
+            s.nv0 = nv;
+            s.dnv = r.v[3];
+            s.w = r.k[3];   // SID; divided by |r|^2 per voxel
+        }
+    }
+    return s;
+}
+
+// Whether the row axis of a view stays parallel to z: then u, w and the
+// weight are constant along a voxel column and only the row coordinate moves.
+template <int KIND, bool CIRC>
+constexpr bool column_invariant()
+{
+    return CIRC || KIND == GCT_PARALLEL_2D || KIND == GCT_CONE_PARALLEL;
+}
+
+// One 16 x 16 x ZU voxel tile: each thread owns one (x, y) column of ZU
+// voxels and accumulates over the views in order, so every voxel sums its
+// views in the reference's order (fdk_axial.cuh:189-193) starting from 0 (or
+// from the partial sums already in the volume when accumulating).
+template <int KIND, int S, bool CIRC, bool F64, int ZU>
+__device__ inline void bp_tile3d(const BpLaunch &a, uint32_t bx, uint32_t by, uint32_t bz, uint32_t tid)
+{
+    using T = typename Stor<S>::T;
+    const uint32_t ix = bx * 16u + (tid & 15u);
+    const uint32_t iy = by * 16u + (tid >> 4);
+    const uint32_t iz0 = bz * ZU;
+    if (ix >= a.nx || iy >= a.ny) return;
+    const uint32_t zn = min(static_cast<uint32_t>(ZU), a.nz - iz0);
+    const size_t slice = static_cast<size_t>(a.nx) * a.ny;
+    float *vp = a.vol + static_cast<size_t>(iz0) * slice + static_cast<size_t>(iy) * a.nx + ix;
+
+    float acc[ZU];
+#pragma unroll
+    for (int k = 0; k < ZU; ++k) acc[k] = (a.accumulate && static_cast<uint32_t>(k) < zn) ? vp[k * slice] : 0.0f;
+
+    const float fx = static_cast<float>(ix), fy = static_cast<float>(iy), fz = static_cast<float>(iz0);
+    const float nuf = static_cast<float>(a.nu), nvf = static_cast<float>(a.nv);
+    const size_t view_elems = static_cast<size_t>(a.rows_pv) * a.pitch;
+    const T *base = static_cast<const T *>(a.filt);
+    const uint32_t vend = a.first_view + a.num_views;
+    for (uint32_t v = a.first_view; v < vend; ++v) {
+        const T *p = base + static_cast<size_t>(v) * view_elems;
+        if constexpr (column_invariant<KIND, CIRC>()) {
+            // set up at z index 0 and step by the absolute index, so the
+            // result does not depend on how z is split into tiles
+            const ColSetup s = setup_column<KIND, F64>(a, v, fx, fy, 0.0f, nuf);
+            const Sample cs = col_sample(s.col, nuf);
+#pragma unroll
+            for (int k = 0; k < ZU; ++k) {
+                const float zk = fz + static_cast<float>(k);
+                const float row = fmaf(zk, s.drow, s.row0);
+                const float val = bilinear<S>(p, a.pitch, cs, row, nvf);
+                float w = s.w;
+                if constexpr (KIND == GCT_CYLINDRICAL) {
+                    const float nvk = fmaf(zk, s.dnv, s.nv0);
+                    w = s.w * __builtin_amdgcn_rcpf(fmaf(nvk, nvk, s.rho2));
+                }
+                acc[k] = fmaf(val, w, acc[k]);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < ZU; ++k) {
+                const ColSetup s = setup_column<KIND, F64>(a, v, fx, fy, fz + static_cast<float>(k), nuf);
+                const float val = bilinear<S>(p, a.pitch, col_sample(s.col, nuf), s.row0, nvf);
+                float w = s.w;
+                if constexpr (KIND == GCT_CYLINDRICAL) w = s.w * __builtin_amdgcn_rcpf(fmaf(s.nv0, s.nv0, s.rho2));
+                acc[k] = fmaf(val, w, acc[k]);
+            }
+        }
+    }
+    const float sc = a.scale;
+#pragma unroll
+    for (int k = 0; k < ZU; ++k)
+        if (static_cast<uint32_t>(k) < zn) vp[k * slice] = a.finish ? acc[k] * sc : acc[k];
+}
+
+// One tile of a 2-D image (1-D detector): pixel (ix, iy) per thread.
+template <int KIND, int S>
+__device__ inline void bp_tile2d(const BpLaunch &a, uint32_t ix, uint32_t iy)
+{
+    using T = typename Stor<S>::T;
+    if (ix >= a.nx || iy >= a.ny) return;
+    float *vp = a.vol + static_cast<size_t>(iy) * a.nx + ix;
+    float acc = a.accumulate ? *vp : 0.0f;
+    const float fx = static_cast<float>(ix), fy = static_cast<float>(iy);
+    const float nuf = static_cast<float>(a.nu);
+    const size_t view_elems = static_cast<size_t>(a.rows_pv) * a.pitch;
+    const T *base = static_cast<const T *>(a.filt);
+    const uint32_t vend = a.first_view + a.num_views;
+    for (uint32_t v = a.first_view; v < vend; ++v) {
+        const BpRec &r = a.rec[v];
+        const T *p = base + static_cast<size_t>(v) * view_elems;
+        float col, w;
+        if constexpr (KIND == GCT_PARALLEL_1D) {
+            col = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
+            w = 1.0f;
+        } else if constexpr (KIND == GCT_FAN_LINE_1D) {
+            const float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
+            const float nu = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
+            const float rd = 1.0f / nw;
+            col = fmaf(nu, rd, r.k[0]);
+            w = rd * rd;
+        } else {   // GCT_FAN_ARC_1D
+            const float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
+            const float nu = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
+            col = fmaf(atan2f(nu, nw), a.inv_dgamma, r.k[0]);
+            w = r.k[3] / fmaf(nu, nu, nw * nw);
+        }
+        const float val = linear<S>(p, a.pitch, col_sample(col, nuf));
+        acc = fmaf(val, w, acc);
+    }
+    *vp = a.finish ? acc * a.scale : acc;
+}
+
+// Blocks are dealt round-robin to the 8 XCDs; renumbering them so that each
+// XCD gets a contiguous run of tiles keeps an XCD's tiles (and the detector
+// rows they read) together in that XCD's L2.
+__device__ inline uint32_t xcd_tile(uint32_t bid, uint32_t nblk)
+{
+    const uint32_t xcd = bid & 7u, local = bid >> 3;
+    const uint32_t q = nblk >> 3, rmd = nblk & 7u;
+    return xcd * q + min(xcd, rmd) + local;
+}
+
+template <int KIND, int S, bool CIRC, bool F64, int ZU>
+__global__ __launch_bounds__(256) void bp3d_kernel(BpLaunch a)
+{
+    const uint32_t nbx = (a.nx + 15u) / 16u, nby = (a.ny + 15u) / 16u;
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t bx = t % nbx, rest = t / nbx;
+    const uint32_t by = rest % nby, bz = rest / nby;
+    bp_tile3d<KIND, S, CIRC, F64, ZU>(a, bx, by, bz, threadIdx.x);
+}
+
+template <int KIND, int S>
+__global__ __launch_bounds__(64) void bp2d_kernel(BpLaunch a)
+{
+    const uint32_t nbx = (a.nx + 7u) / 8u;
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t bx = t % nbx, by = t / nbx;
+    bp_tile2d<KIND, S>(a, bx * 8u + (threadIdx.x & 7u), by * 8u + (threadIdx.x >> 3));
+}
+
+// ---- LDS-staged backprojection ----------------------------------------------
+//
+// A workgroup owns a 16 x 16 x ZU voxel tile.  For each block of VB views it
+// projects the tile's 8 corner voxels, takes the bounding box of the detector
+// pixels they reach (plus one pixel of margin), and stages that box from HBM
+// into LDS as pairs (p[r][c], p[r][c+1] - p[r][c]) -- value and horizontal
+// difference, FP16/BF16 widened to FP32 -- so each thread's bilinear tap is
+// two 64-bit LDS reads and one FMA per row.  The differences are the same
+// FP32 subtractions the direct kernel performs per update, so both kernels
+// give the same bits.  Coordinates are clamped to the staged box: the box never
+// extends past the zero padding of the filtered-projection buffer, so the
+// clamp also reproduces the border-zero texture, and it keeps every LDS read
+// inside the box.  A view whose box does not fit is read from HBM directly.
+
+constexpr int kTP = 48;   // staged (value, difference) pairs per row
+
+struct Box {
+    int clo, rlo, ncols, nrows;   // ncols == 0: the view is read from HBM
+};
+
+template <int KIND, int ZU>
+__device__ inline void tile_boxes(const BpLaunch &a, uint32_t bx, uint32_t by, uint32_t bz, uint32_t vb0,
+                                  uint32_t nvb, int TR, Box *boxes, uint32_t tid)
+{
+    if (tid >= 8u * nvb) return;
+    const uint32_t j = tid >> 3, corner = tid & 7u;
+    const uint32_t cx = min(bx * 16u + ((corner & 1u) ? 15u : 0u), a.nx - 1u);
+    const uint32_t cy = min(by * 16u + ((corner & 2u) ? 15u : 0u), a.ny - 1u);
+    const uint32_t cz = min(bz * ZU + ((corner & 4u) ? static_cast<uint32_t>(ZU - 1) : 0u), a.nz - 1u);
+    const float nuf = static_cast<float>(a.nu);
+    const ColSetup s = setup_column<KIND, false>(a, vb0 + j, static_cast<float>(cx), static_cast<float>(cy), 0.0f, nuf);
+    const float row = fmaf(static_cast<float>(cz), s.drow, s.row0);
+    float cmin = s.col, cmax = s.col, rmin = row, rmax = row;
+#pragma unroll
+    for (int off = 1; off < 8; off <<= 1) {
+        cmin = fminf(cmin, __shfl_xor(cmin, off));
+        cmax = fmaxf(cmax, __shfl_xor(cmax, off));
+        rmin = fminf(rmin, __shfl_xor(rmin, off));
+        rmax = fmaxf(rmax, __shfl_xor(rmax, off));
+    }
+    if (corner != 0u) return;
+    Box b{0, 0, 0, 0};
+    const bool finite = cmin > -1.0e6f && cmax < 1.0e6f && rmin > -1.0e6f && rmax < 1.0e6f;
+    if (finite) {
+        const int clo = max(static_cast<int>(floorf(cmin)) - 1, -1);
+        const int chi = min(static_cast<int>(floorf(cmax)) + 2, static_cast<int>(a.nu) + 1);
+        const int rlo = max(static_cast<int>(floorf(rmin)) - 1, -1);
+        const int rhi = min(static_cast<int>(floorf(rmax)) + 2, static_cast<int>(a.nv) + 1);
+        const int nc = chi - clo + 1, nr = rhi - rlo + 1;
+        if (nc >= 2 && nr >= 2 && nc - 1 <= kTP && nr <= TR) b = Box{clo, rlo, nc, nr};
+    }
+    boxes[j] = b;
+}
+
+// One 16 x 16 x ZU tile (bx, by, bz) of a circular 3-D scan; every thread of
+// the 256-thread workgroup calls it (it holds workgroup barriers).  `tile`
+// and `boxes` are the workgroup's LDS staging area of VB * TR * kTP pairs and
+// VB boxes.
+template <int KIND, int S, bool F64, int ZU, int TR, int VB>
+__device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint32_t bz, uint32_t tid, float2 *tile,
+                              Box *boxes)
+{
+    using T = typename Stor<S>::T;
+    const uint32_t ix = bx * 16u + (tid & 15u), iy = by * 16u + (tid >> 4), iz0 = bz * ZU;
+    const bool active = ix < a.nx && iy < a.ny;
+    const uint32_t zn = min(static_cast<uint32_t>(ZU), a.nz - iz0);
+    const size_t slice = static_cast<size_t>(a.nx) * a.ny;
+    float *vp = a.vol + static_cast<size_t>(iz0) * slice + static_cast<size_t>(active ? iy : 0u) * a.nx + (active ? ix : 0u);
+
+    float acc[ZU];
+#pragma unroll
+    for (int k = 0; k < ZU; ++k)
+        acc[k] = (active && a.accumulate && static_cast<uint32_t>(k) < zn) ? vp[k * slice] : 0.0f;
+
+    const float fx = static_cast<float>(ix), fy = static_cast<float>(iy), fz = static_cast<float>(iz0);
+    const float nuf = static_cast<float>(a.nu), nvf = static_cast<float>(a.nv);
+    const size_t view_elems = static_cast<size_t>(a.rows_pv) * a.pitch;
+    const T *base = static_cast<const T *>(a.filt);
+    const uint32_t vend = a.first_view + a.num_views;
+
+    for (uint32_t vb0 = a.first_view; vb0 < vend; vb0 += VB) {
+        const uint32_t nvb = min(static_cast<uint32_t>(VB), vend - vb0);
+        tile_boxes<KIND, ZU>(a, bx, by, bz, vb0, nvb, TR, boxes, tid);
+        __syncthreads();
+        for (uint32_t j = 0; j < nvb; ++j) {
+            const Box b = boxes[j];
+            if (b.ncols == 0) continue;
+            const T *p = base + static_cast<size_t>(vb0 + j) * view_elems;
+            float2 *tl = tile + j * (TR * kTP);
+            const int m = b.ncols - 1;            // staged pairs per row
+            const bool narrow = m <= 32;          // two rows per wave
+            const int c = narrow ? static_cast<int>(tid & 31u) : static_cast<int>(tid & 63u);
+            const int rstep = narrow ? 8 : 4;
+            if (c < m) {
+                for (int r = narrow ? static_cast<int>(tid >> 5) : static_cast<int>(tid >> 6); r < b.nrows; r += rstep) {
+                    const uint32_t g = static_cast<uint32_t>(b.rlo + r + 1) * a.pitch + static_cast<uint32_t>(b.clo + c + 1);
+                    const float v0 = Stor<S>::ld(p, g), v1 = Stor<S>::ld(p, g + 1u);
+                    tl[r * kTP + c] = make_float2(v0, v1 - v0);
+                }
+            }
+        }
+        __syncthreads();
+        if (active) {
+            for (uint32_t j = 0; j < nvb; ++j) {
+                const uint32_t v = vb0 + j;
+                const ColSetup s = setup_column<KIND, F64>(a, v, fx, fy, 0.0f, nuf);
+                const Box b = boxes[j];
+                if (b.ncols != 0) {
+                    const float cl = __builtin_amdgcn_fmed3f(s.col, static_cast<float>(b.clo),
+                                                             static_cast<float>(b.clo + b.ncols - 2));
+                    const float cf = floorf(cl);
+                    const float alpha = cl - cf;
+                    // Pair (r, c) of the box sits at byte 8 (j TR kTP + (r - rlo) kTP + (c - clo)).
+                    // The per-view terms fold into one base; the row term is added in float,
+                    // which is exact because every offset is an integer below 2^24.
+                    // (signed: the base is negative when the box starts below row 0 of view j's slot)
+                    const int ebase = static_cast<int>(j) * (TR * kTP) + (static_cast<int>(cf) - b.clo) - b.rlo * kTP;
+                    const float obase = static_cast<float>(8 * ebase);
+                    const float rlo = static_cast<float>(b.rlo), rhi = static_cast<float>(b.rlo + b.nrows - 2);
+#pragma unroll
+                    for (int k = 0; k < ZU; ++k) {
+                        const float zk = fz + static_cast<float>(k);
+                        const float row = __builtin_amdgcn_fmed3f(fmaf(zk, s.drow, s.row0), rlo, rhi);
+                        const float rf = floorf(row);
+                        const float beta = row - rf;
+                        const uint32_t off = static_cast<uint32_t>(fmaf(rf, static_cast<float>(8 * kTP), obase));
+                        const float2 *q = reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(tile) + off);
+                        const float2 tp = q[0], bt = q[kTP];
+                        const float top = fmaf(alpha, tp.y, tp.x);
+                        const float bot = fmaf(alpha, bt.y, bt.x);
+                        const float val = fmaf(beta, bot - top, top);
+                        float w = s.w;
+                        if constexpr (KIND == GCT_CYLINDRICAL) {
+                            const float nvk = fmaf(zk, s.dnv, s.nv0);
+                            w = s.w * __builtin_amdgcn_rcpf(fmaf(nvk, nvk, s.rho2));
+                        }
+                        acc[k] = fmaf(val, w, acc[k]);
+                    }
+                } else {
+                    const T *p = base + static_cast<size_t>(v) * view_elems;
+                    const Sample cs = col_sample(s.col, nuf);
+#pragma unroll
+                    for (int k = 0; k < ZU; ++k) {
+                        const float zk = fz + static_cast<float>(k);
+                        const float row = fmaf(zk, s.drow, s.row0);
+                        const float val = bilinear<S>(p, a.pitch, cs, row, nvf);
+                        float w = s.w;
+                        if constexpr (KIND == GCT_CYLINDRICAL) {
+                            const float nvk = fmaf(zk, s.dnv, s.nv0);
+                            w = s.w * __builtin_amdgcn_rcpf(fmaf(nvk, nvk, s.rho2));
+                        }
+                        acc[k] = fmaf(val, w, acc[k]);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!active) return;
+    const float sc = a.scale;
+#pragma unroll
+    for (int k = 0; k < ZU; ++k)
+        if (static_cast<uint32_t>(k) < zn) vp[k * slice] = a.finish ? acc[k] * sc : acc[k];
+}
+
+template <int KIND, int S, bool F64, int ZU, int TR, int VB>
+__global__ __launch_bounds__(256) void bp3d_lds_kernel(BpLaunch a)
+{
+    __shared__ float2 tile[VB * TR * kTP];
+    __shared__ Box boxes[VB];
+    const uint32_t nbx = (a.nx + 15u) / 16u, nby = (a.ny + 15u) / 16u;
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t bx = t % nbx, rest = t / nbx;
+    bp_tile3d_lds<KIND, S, F64, ZU, TR, VB>(a, bx, rest % nby, rest / nby, threadIdx.x, tile, boxes);
+}
+
+// 2-D images: each block of views has its (padded) detector rows staged whole
+// in LDS; a 16 x 16 pixel tile per workgroup.
+template <int KIND, int S>
+__global__ __launch_bounds__(256) void bp2d_lds_kernel(BpLaunch a, uint32_t vb)
+{
+    using T = typename Stor<S>::T;
+    extern __shared__ float rows[];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nbx = (a.nx + 15u) / 16u;
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t ix = (t % nbx) * 16u + (tid & 15u), iy = (t / nbx) * 16u + (tid >> 4);
+    const bool active = ix < a.nx && iy < a.ny;
+    float *vp = a.vol + static_cast<size_t>(active ? iy : 0u) * a.nx + (active ? ix : 0u);
+    float acc = (active && a.accumulate) ? *vp : 0.0f;
+    const float fx = static_cast<float>(ix), fy = static_cast<float>(iy);
+    const float nuf = static_cast<float>(a.nu);
+    const size_t view_elems = static_cast<size_t>(a.rows_pv) * a.pitch;
+    const T *base = static_cast<const T *>(a.filt);
+    const uint32_t vend = a.first_view + a.num_views;
+    for (uint32_t vb0 = a.first_view; vb0 < vend; vb0 += vb) {
+        const uint32_t nvb = min(vb, vend - vb0);
+        for (uint32_t j = 0; j < nvb; ++j) {
+            const T *p = base + static_cast<size_t>(vb0 + j) * view_elems + a.pitch;
+            for (uint32_t c = tid; c < a.pitch; c += 256u) rows[j * a.pitch + c] = Stor<S>::ld(p, c);
+        }
+        __syncthreads();
+        if (active) {
+            for (uint32_t j = 0; j < nvb; ++j) {
+                const BpRec &r = a.rec[vb0 + j];
+                float col, w;
+                if constexpr (KIND == GCT_PARALLEL_1D) {
+                    col = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
+                    w = 1.0f;
+                } else if constexpr (KIND == GCT_FAN_LINE_1D) {
+                    const float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
+                    const float nu = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
+                    const float rd = 1.0f / nw;
+                    col = fmaf(nu, rd, r.k[0]);
+                    w = rd * rd;
+                } else {
+                    const float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
+                    const float nu = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
+                    col = fmaf(atan2f(nu, nw), a.inv_dgamma, r.k[0]);
+                    w = r.k[3] / fmaf(nu, nu, nw * nw);
+                }
+                const Sample cs = col_sample(col, nuf);
+                const float *rr = rows + j * a.pitch + cs.colbase;
+                const float t0 = rr[0], t1 = rr[1];
+                acc = fmaf(fmaf(cs.alpha, t1 - t0, t0), w, acc);
+            }
+        }
+        __syncthreads();
+    }
+    if (active) *vp = a.finish ? acc * a.scale : acc;
+}
+
+// ---- persistent batched backprojection -------------------------------------
+
+constexpr int kBatchZU = static_cast<int>(kBatchDirectZU);   // direct tiles (general trajectories)
+constexpr int kBatchTR = 40, kBatchVB = 3;                    // LDS tiles (circular), depth kBatchLdsZU
+
+// Persistent backprojection over the scans of one group (same geometry kind,
+// storage and tile type; a batch launches one such kernel per group).  Work
+// items are (scan, tile) pairs numbered scan by scan; each workgroup takes
+// items from the group's device counter until none are left, so the grid
+// drains on its own whatever the mix of scan sizes.
+template <int KIND, int S, bool LDS>
+__global__ __launch_bounds__(256) void bp_batch_kernel(const BatchScan *__restrict__ scans, uint32_t nscans,
+                                                       uint32_t total, uint32_t *counter)
+{
+    constexpr bool k1d = KIND <= GCT_FAN_LINE_1D;
+    __shared__ uint32_t item;
+    __shared__ float2 tile[LDS ? kBatchVB * kBatchTR * kTP : 1];
+    __shared__ Box boxes[kBatchVB];
+    const uint32_t tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) item = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t t = item;
+        __syncthreads();
+        if (t >= total) break;
+        uint32_t lo = 0, hi = nscans;   // last scan with tile_begin <= t
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (scans[mid].tile_begin <= t) lo = mid; else hi = mid;
+        }
+        const BatchScan &sc = scans[lo];
+        const BpLaunch &a = sc.bp;
+        const uint32_t local = t - sc.tile_begin;
+        if constexpr (k1d) {
+            const uint32_t bx = local % sc.tiles_x, by = local / sc.tiles_x;
+            bp_tile2d<KIND, S>(a, bx * 16u + (tid & 15u), by * 16u + (tid >> 4));
+        } else {
+            const uint32_t bx = local % sc.tiles_x, rest = local / sc.tiles_x;
+            const uint32_t by = rest % sc.tiles_y, bz = rest / sc.tiles_y;
+            if constexpr (LDS)
+                bp_tile3d_lds<KIND, S, false, static_cast<int>(kBatchLdsZU), kBatchTR, kBatchVB>(a, bx, by, bz, tid,
+                                                                                                 tile, boxes);
+            else
+                bp_tile3d<KIND, S, false, false, kBatchZU>(a, bx, by, bz, tid);
+        }
+    }
+}
+
+// ---- launch helpers ---------------------------------------------------------
+
+template <int KIND, int S, bool CIRC, bool F64>
+void launch_bp3d_zu(const BpLaunch &a, hipStream_t s)
+{
+    const uint32_t nbx = (a.nx + 15u) / 16u, nby = (a.ny + 15u) / 16u;
+    const uint32_t zu = a.z_unroll;
+    const uint32_t nbz = (a.nz + zu - 1u) / zu;
+    const uint32_t blocks = nbx * nby * nbz;
+    switch (zu) {
+    case 4: hipLaunchKernelGGL((bp3d_kernel<KIND, S, CIRC, F64, 4>), dim3(blocks), dim3(256), 0, s, a); break;
+    case 16: hipLaunchKernelGGL((bp3d_kernel<KIND, S, CIRC, F64, 16>), dim3(blocks), dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL((bp3d_kernel<KIND, S, CIRC, F64, 8>), dim3(blocks), dim3(256), 0, s, a); break;
+    }
+}
+
+template <int KIND, int S, bool F64>
+void launch_bp3d_lds(const BpLaunch &a, hipStream_t s)
+{
+    const uint32_t nbx = (a.nx + 15u) / 16u, nby = (a.ny + 15u) / 16u;
+    const uint32_t zu = (a.z_unroll == 32u) ? 32u : 16u;
+    const uint32_t blocks = nbx * nby * ((a.nz + zu - 1u) / zu);
+    if (zu == 32u) hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 32, 64, 2>), dim3(blocks), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 3>), dim3(blocks), dim3(256), 0, s, a);
+}
+
+template <int KIND, int S>
+void launch_bp3d_variant(const BpLaunch &a, hipStream_t s)
+{
+    const bool lds = a.variant == GCT_BP_LDS;
+    if constexpr (KIND == GCT_FLAT || KIND == GCT_CYLINDRICAL) {
+        if (lds && a.circular) {
+            if (a.setup_f64) launch_bp3d_lds<KIND, S, true>(a, s);
+            else launch_bp3d_lds<KIND, S, false>(a, s);
+        } else if (a.setup_f64) {
+            if (a.circular) launch_bp3d_zu<KIND, S, true, true>(a, s);
+            else launch_bp3d_zu<KIND, S, false, true>(a, s);
+        } else {
+            if (a.circular) launch_bp3d_zu<KIND, S, true, false>(a, s);
+            else launch_bp3d_zu<KIND, S, false, false>(a, s);
+        }
+    } else {
+        if (lds) launch_bp3d_lds<KIND, S, false>(a, s);
+        else launch_bp3d_zu<KIND, S, true, false>(a, s);
+    }
+}
+
+template <int KIND, int S>
+void launch_bp2d(const BpLaunch &a, hipStream_t s)
+{
+    if (a.variant == GCT_BP_LDS) {
+        const uint32_t vb = std::max(1u, std::min(16u, 32768u / (a.pitch * 4u)));
+        const uint32_t blocks = ((a.nx + 15u) / 16u) * ((a.ny + 15u) / 16u);
+        hipLaunchKernelGGL((bp2d_lds_kernel<KIND, S>), dim3(blocks), dim3(256), vb * a.pitch * sizeof(float), s, a, vb);
+    } else {
+        const uint32_t blocks = ((a.nx + 7u) / 8u) * ((a.ny + 7u) / 8u);
+        hipLaunchKernelGGL((bp2d_kernel<KIND, S>), dim3(blocks), dim3(64), 0, s, a);
+    }
+}
+template <int KIND, int S>
+void launch_batch_kind(const BatchScan *scans, const BatchGroup &g, uint32_t *counter, uint32_t grid, hipStream_t s)
+{
+    if constexpr (KIND <= GCT_FAN_LINE_1D) {
+        hipLaunchKernelGGL((bp_batch_kernel<KIND, S, false>), dim3(grid), dim3(256), 0, s, scans, g.num_scans,
+                           g.total_tiles, counter);
+    } else if constexpr (KIND == GCT_FLAT || KIND == GCT_CYLINDRICAL) {
+        if (g.lds)
+            hipLaunchKernelGGL((bp_batch_kernel<KIND, S, true>), dim3(grid), dim3(256), 0, s, scans, g.num_scans,
+                               g.total_tiles, counter);
+        else
+            hipLaunchKernelGGL((bp_batch_kernel<KIND, S, false>), dim3(grid), dim3(256), 0, s, scans, g.num_scans,
+                               g.total_tiles, counter);
+    } else {
+        hipLaunchKernelGGL((bp_batch_kernel<KIND, S, true>), dim3(grid), dim3(256), 0, s, scans, g.num_scans,
+                           g.total_tiles, counter);
+    }
+}
+
+// Backprojection of one geometry kind (all storage types).
+template <int KIND>
+void backproject_kind(const BpLaunch &a, hipStream_t s)
+{
+    switch (a.storage) {
+    case GCT_STORE_F16:
+        if constexpr (KIND <= GCT_FAN_LINE_1D) launch_bp2d<KIND, GCT_STORE_F16>(a, s);
+        else launch_bp3d_variant<KIND, GCT_STORE_F16>(a, s);
+        break;
+    case GCT_STORE_BF16:
+        if constexpr (KIND <= GCT_FAN_LINE_1D) launch_bp2d<KIND, GCT_STORE_BF16>(a, s);
+        else launch_bp3d_variant<KIND, GCT_STORE_BF16>(a, s);
+        break;
+    default:
+        if constexpr (KIND <= GCT_FAN_LINE_1D) launch_bp2d<KIND, GCT_STORE_F32>(a, s);
+        else launch_bp3d_variant<KIND, GCT_STORE_F32>(a, s);
+        break;
+    }
+}
+
+// Persistent batch kernel of one geometry kind (all storage types).
+template <int KIND>
+void batch_group_kind(const BatchScan *scans, const BatchGroup &g, uint32_t *counter, uint32_t grid, hipStream_t s)
+{
+    switch (g.storage) {
+    case GCT_STORE_F16: launch_batch_kind<KIND, GCT_STORE_F16>(scans, g, counter, grid, s); break;
+    case GCT_STORE_BF16: launch_batch_kind<KIND, GCT_STORE_BF16>(scans, g, counter, grid, s); break;
+    default: launch_batch_kind<KIND, GCT_STORE_F32>(scans, g, counter, grid, s); break;
+    }
+}
+
+}  // namespace
+}  // namespace gct

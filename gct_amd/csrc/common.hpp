@@ -186,6 +186,19 @@ struct BatchGroup {
     uint32_t first_scan, num_scans, total_tiles;
 };
 void launch_backproject_batch_group(const BatchScan *scans_dev, const BatchGroup &g, uint32_t *counter, hipStream_t s);
+
+// Per-geometry-family instantiations (bp_1d.hip, bp_parallel.hip, bp_cyl.hip,
+// bp_flat.hip); `scans` points at the group's first scan.
+void launch_backproject_1d(const BpLaunch &a, hipStream_t s);
+void launch_backproject_parallel(const BpLaunch &a, hipStream_t s);
+void launch_backproject_cyl(const BpLaunch &a, hipStream_t s);
+void launch_backproject_flat(const BpLaunch &a, hipStream_t s);
+void launch_batch_group_1d(const BatchScan *scans, const BatchGroup &g, uint32_t *counter, uint32_t grid, hipStream_t s);
+void launch_batch_group_parallel(const BatchScan *scans, const BatchGroup &g, uint32_t *counter, uint32_t grid,
+                                 hipStream_t s);
+void launch_batch_group_cyl(const BatchScan *scans, const BatchGroup &g, uint32_t *counter, uint32_t grid, hipStream_t s);
+void launch_batch_group_flat(const BatchScan *scans, const BatchGroup &g, uint32_t *counter, uint32_t grid,
+                             hipStream_t s);
 void launch_zero_counters(uint32_t *counters, uint32_t n, hipStream_t s);
 
 }  // namespace gct
