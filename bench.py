@@ -11,8 +11,8 @@ flat      The headline number.  A step is one plain FDK reconstruction of the
           backprojection, projections already resident in HBM.  The plan is
           autotuned once before the warm-up.  `value` is voxel x view updates
           per second over the timed steps; the backprojection kernel's
-          roofline is measured with HIP events on its stream.  Beside it,
-          the same backprojection launched one view at a time with a
+          roofline is measured with HIP events on its stream.
+perview   The same backprojection launched one view at a time with a
           full-volume read-modify-write per view (the reference's structure).
 precision The short-scan C-arm workload (512^3, 248 views over 200 deg) with
           FP32, FP16 and BF16 filtered-projection storage.
@@ -29,7 +29,7 @@ served    The serving stream: 32 seeded requests of mixed geometry and size,
           requests queued at once: capacity), then with Poisson arrivals at
           80 % of that capacity (latency).
 
-    python bench.py [--steps K] [--warmup W] [--no-cpu-baseline] [--sections flat,precision,batched,served]
+    python bench.py [--steps K] [--warmup W] [--no-cpu-baseline] [--sections flat,perview,precision,batched,served]
 """
 from __future__ import annotations
 
@@ -96,7 +96,7 @@ def cpu_baseline(wl, proj, gpu_vol, nthreads):
 
 # ---- flat-cone plain FDK (the headline) -----------------------------------------
 
-def run_flat(a, g, W, torch, np):
+def run_flat(a, g, W, torch, np, sections):
     wl = W.flat_cone_256()
     t = time.perf_counter()
     proj = wl.projections(a.threads)
@@ -146,20 +146,6 @@ def run_flat(a, g, W, torch, np):
         except (OSError, ValueError):
             traffic = None
 
-    # The reference's structure on the same kernels: one view per launch, each
-    # launch reading and writing the whole volume (fdk_axial.cuh:189-199).
-    V = wl.scan.num_proj
-    per_view = []
-    for rep in range(2):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for i in range(V):
-            plan.backproject(v, first=i, n=1, accumulate=i > 0, finish=i == V - 1, stream=stream)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        per_view.append(e0.elapsed_time(e1))
-    per_view_ms = min(per_view)
-
     vol = v.cpu().numpy()
     truth = g.phantom_rasterize(wl.img, wl.phantom)
     rmse_truth = float(np.sqrt(np.mean((vol - truth) ** 2)))
@@ -192,12 +178,6 @@ def run_flat(a, g, W, torch, np):
         "bp_ms": round(bp_mean, 4),
         "latency_ms": {"p50": round(_percentile(step_ms, 0.5), 4), "p99": round(_percentile(step_ms, 0.99), 4)},
         "rmse_vs_phantom": rmse_truth,
-        "reference_structured_bp": {
-            "what": "same kernels launched one view at a time, full-volume read-modify-write per view",
-            "bp_ms": round(per_view_ms, 3),
-            "bp_gups": round(upd / (per_view_ms * 1e-3) / 1e9, 2),
-            "fused_vs_per_view": round(per_view_ms / bp_mean, 3),
-        },
         "roofline": {
             "bound": "valu",
             "kernel": kernel or f"bp3d_{variant}_kernel<FLAT, f32, ZU={info.z_unroll}>",
@@ -212,6 +192,26 @@ def run_flat(a, g, W, torch, np):
         },
         "input_generation_s": round(t_gen, 2),
     }
+    if "perview" in sections:
+        # The reference's structure on the same kernels: one view per launch,
+        # each launch reading and writing the whole volume (fdk_axial.cuh:189-199).
+        V = wl.scan.num_proj
+        per_view = []
+        for rep in range(2):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for i in range(V):
+                plan.backproject(v, first=i, n=1, accumulate=i > 0, finish=i == V - 1, stream=stream)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            per_view.append(e0.elapsed_time(e1))
+        per_view_ms = min(per_view)
+        out["reference_structured_bp"] = {
+            "what": "same kernel launched one view at a time, full-volume read-modify-write per view",
+            "bp_ms": round(per_view_ms, 3),
+            "bp_gups": round(upd / (per_view_ms * 1e-3) / 1e9, 2),
+            "fused_vs_per_view": round(per_view_ms / bp_mean, 3),
+        }
     del p, v
     plan.close()
     return out, wl, proj, vol
@@ -427,7 +427,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--sections", default="flat,precision,batched,served")
+    ap.add_argument("--sections", default="flat,perview,precision,batched,served")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy 5 times (reference kernel for HBM byte counters)")
@@ -456,7 +456,7 @@ def main():
         print(json.dumps({"copy_calibration_bytes_per_launch": {"read": x.numel() * 4, "write": y.numel() * 4}}))
         return
 
-    out, wl, proj, vol = run_flat(a, g, W, torch, np)
+    out, wl, proj, vol = run_flat(a, g, W, torch, np, sections)
     if "precision" in sections:
         print("bench: precision section", file=sys.stderr, flush=True)
         out["precision"] = run_precision(a, g, W, torch, np)

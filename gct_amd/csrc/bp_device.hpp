@@ -387,18 +387,32 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
         for (uint32_t j = 0; j < nvb; ++j) {
             const Box b = boxes[j];
             if (b.ncols == 0) continue;
-            const T *p = base + static_cast<size_t>(vb0 + j) * view_elems;
+            // Box row r, column c is padded pixel (rlo + 1 + r, clo + 1 + c): one
+            // uniform base per view, 32-bit offsets.  A row of m pairs needs m + 1
+            // values; each lane loads one and takes its right neighbour from the
+            // next lane, and every load of the view is issued before the first use.
+            const T *p = base + static_cast<size_t>(vb0 + j) * view_elems +
+                         (static_cast<size_t>(b.rlo + 1) * a.pitch + static_cast<uint32_t>(b.clo + 1));
             float2 *tl = tile + j * (TR * kTP);
             const int m = b.ncols - 1;            // staged pairs per row
-            const bool narrow = m <= 32;          // two rows per wave
+            const bool narrow = m < 32;           // two rows per wave, 32 lanes each
             const int c = narrow ? static_cast<int>(tid & 31u) : static_cast<int>(tid & 63u);
+            const int r0 = narrow ? static_cast<int>(tid >> 5) : static_cast<int>(tid >> 6);
             const int rstep = narrow ? 8 : 4;
-            if (c < m) {
-                for (int r = narrow ? static_cast<int>(tid >> 5) : static_cast<int>(tid >> 6); r < b.nrows; r += rstep) {
-                    const uint32_t g = static_cast<uint32_t>(b.rlo + r + 1) * a.pitch + static_cast<uint32_t>(b.clo + c + 1);
-                    const float v0 = Stor<S>::ld(p, g), v1 = Stor<S>::ld(p, g + 1u);
-                    tl[r * kTP + c] = make_float2(v0, v1 - v0);
-                }
+            constexpr int kIt = (TR + 3) / 4;     // row steps of the wide mode (the narrow mode uses half)
+            float v[kIt];
+#pragma unroll
+            for (int it = 0; it < kIt; ++it) {
+                const int r = r0 + it * rstep;
+                v[it] = (r < b.nrows && c <= m)
+                            ? Stor<S>::ld(p, static_cast<uint32_t>(r) * a.pitch + static_cast<uint32_t>(c))
+                            : 0.0f;
+            }
+#pragma unroll
+            for (int it = 0; it < kIt; ++it) {
+                const int r = r0 + it * rstep;
+                const float right = __shfl_down(v[it], 1);
+                if (r < b.nrows && c < m) tl[r * kTP + c] = make_float2(v[it], right - v[it]);
             }
         }
         __syncthreads();
@@ -465,8 +479,21 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
         if (static_cast<uint32_t>(k) < zn) vp[k * slice] = a.finish ? acc[k] * sc : acc[k];
 }
 
+// Occupancy target of an LDS kernel variant: workgroups (one wave per SIMD
+// each) that fit the 160 KiB of LDS per CU, capped by what its registers
+// allow without spilling (z-unroll 16: 5, i.e. <= 96 VGPRs; 32: 3).
+template <int ZU, int TR, int VB>
+constexpr int lds_waves_per_simd()
+{
+    constexpr int bytes = VB * TR * kTP * 8 + VB * static_cast<int>(sizeof(Box));
+    constexpr int by_lds = (160 * 1024) / bytes;
+    constexpr int by_regs = ZU >= 32 ? 3 : 5;
+    return by_lds < 1 ? 1 : (by_lds < by_regs ? by_lds : by_regs);
+}
+
 template <int KIND, int S, bool F64, int ZU, int TR, int VB>
-__global__ __launch_bounds__(256) void bp3d_lds_kernel(BpLaunch a)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lds_waves_per_simd<ZU, TR, VB>(), 8)))
+void bp3d_lds_kernel(BpLaunch a)
 {
     __shared__ float2 tile[VB * TR * kTP];
     __shared__ Box boxes[VB];
@@ -535,7 +562,7 @@ __global__ __launch_bounds__(256) void bp2d_lds_kernel(BpLaunch a, uint32_t vb)
 // ---- persistent batched backprojection -------------------------------------
 
 constexpr int kBatchZU = static_cast<int>(kBatchDirectZU);   // direct tiles (general trajectories)
-constexpr int kBatchTR = 40, kBatchVB = 3;                    // LDS tiles (circular), depth kBatchLdsZU
+constexpr int kBatchTR = 40, kBatchVB = 2;                    // LDS tiles (circular), depth kBatchLdsZU
 
 // Persistent backprojection over the scans of one group (same geometry kind,
 // storage and tile type; a batch launches one such kernel per group).  Work
@@ -543,7 +570,9 @@ constexpr int kBatchTR = 40, kBatchVB = 3;                    // LDS tiles (circ
 // items from the group's device counter until none are left, so the grid
 // drains on its own whatever the mix of scan sizes.
 template <int KIND, int S, bool LDS>
-__global__ __launch_bounds__(256) void bp_batch_kernel(const BatchScan *__restrict__ scans, uint32_t nscans,
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(LDS ? lds_waves_per_simd<static_cast<int>(kBatchLdsZU), kBatchTR, kBatchVB>() : 1, 8)))
+void bp_batch_kernel(const BatchScan *__restrict__ scans, uint32_t nscans,
                                                        uint32_t total, uint32_t *counter)
 {
     constexpr bool k1d = KIND <= GCT_FAN_LINE_1D;
@@ -602,8 +631,14 @@ void launch_bp3d_lds(const BpLaunch &a, hipStream_t s)
     const uint32_t nbx = (a.nx + 15u) / 16u, nby = (a.ny + 15u) / 16u;
     const uint32_t zu = (a.z_unroll == 32u) ? 32u : 16u;
     const uint32_t blocks = nbx * nby * ((a.nz + zu - 1u) / zu);
-    if (zu == 32u) hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 32, 64, 2>), dim3(blocks), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 3>), dim3(blocks), dim3(256), 0, s, a);
+    const dim3 g(blocks), b(256);
+    switch (zu * 8u + lds_view_block(zu, a.view_block)) {
+    case 32u * 8u + 1u: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 32, 64, 1>), g, b, 0, s, a); break;
+    case 32u * 8u + 2u: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 32, 64, 2>), g, b, 0, s, a); break;
+    case 16u * 8u + 3u: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 3>), g, b, 0, s, a); break;
+    case 16u * 8u + 4u: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 4>), g, b, 0, s, a); break;
+    default: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 2>), g, b, 0, s, a); break;
+    }
 }
 
 template <int KIND, int S>

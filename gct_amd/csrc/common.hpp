@@ -139,7 +139,19 @@ struct BpLaunch {
     bool accumulate, finish;
     uint32_t z_unroll;
     int variant;                  // GCT_BP_DIRECT or GCT_BP_LDS
+    uint32_t view_block;          // LDS kernel: views staged per block (0: default for z_unroll)
 };
+
+// LDS kernel view-block depths per z-unroll: {16: 2, 3, 4 (default 2)}, {32: 1, 2 (default 2)}.
+inline uint32_t lds_view_block(uint32_t z_unroll, uint32_t vb)
+{
+    if (z_unroll == 32u) return (vb == 1u) ? 1u : 2u;
+    return (vb == 3u || vb == 4u) ? vb : 2u;
+}
+inline bool lds_view_block_valid(uint32_t z_unroll, uint32_t vb)
+{
+    return vb == 0u || (z_unroll == 32u ? (vb == 1u || vb == 2u) : (vb >= 2u && vb <= 4u));
+}
 
 struct FpLaunch {
     int kind;

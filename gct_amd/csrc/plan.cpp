@@ -61,6 +61,7 @@ void fft64(std::vector<std::complex<double>> &a)
 struct Tuning {
     uint32_t z_unroll;
     int variant;
+    uint32_t view_block;
 };
 std::mutex g_tune_mu;
 std::map<std::tuple<int, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, int, int, int>, Tuning> g_tune;
@@ -214,6 +215,8 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
     if (o.z_unroll != 0 && !(variant == GCT_BP_DIRECT ? (o.z_unroll == 4 || o.z_unroll == 8 || o.z_unroll == 16)
                                                       : (o.z_unroll == 16 || o.z_unroll == 32)))
         fail(GCT_ERR_INVALID_ARGUMENT, "z_unroll must be 4, 8 or 16 (direct kernel) or 16 or 32 (LDS kernel)");
+    if (!lds_view_block_valid(o.z_unroll ? o.z_unroll : 16u, o.view_block))
+        fail(GCT_ERR_INVALID_ARGUMENT, "view_block must be 2, 3 or 4 (z_unroll 16) or 1 or 2 (z_unroll 32)");
     auto p = std::make_unique<gct_plan>();
     p->device = o.device;
     p->stream = stream;
@@ -306,6 +309,7 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
 
     p->variant = variant;
     p->z_unroll = o.z_unroll ? o.z_unroll : (variant == GCT_BP_LDS ? 16u : 8u);
+    p->view_block = o.view_block;
     if (!o.z_unroll && o.bp_kernel == GCT_BP_AUTO) {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find({geom.kind, img.x.num_samples, img.y.num_samples, img.dim == 3 ? img.z.num_samples : 1u,
@@ -313,6 +317,7 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
         if (it != g_tune.end()) {
             p->z_unroll = it->second.z_unroll;
             p->variant = it->second.variant;
+            p->view_block = it->second.view_block;
         }
     }
     return p;
@@ -375,6 +380,7 @@ BpLaunch bp_args(gct_plan &p, float *vol, uint32_t first, uint32_t n, bool accum
     a.finish = finish;
     a.z_unroll = p.z_unroll;
     a.variant = p.variant;
+    a.view_block = p.view_block;
     return a;
 }
 
@@ -522,6 +528,7 @@ extern "C" gct_status gct_plan_get_info(const gct_plan *p, gct_plan_info *info)
         info->circular = p->circular;
         info->storage = p->opts.storage;
         info->bp_kernel = p->variant;
+        info->view_block = p->variant == GCT_BP_LDS ? lds_view_block(p->z_unroll, p->view_block) : 0u;
     });
 }
 
@@ -695,39 +702,41 @@ extern "C" gct_status gct_plan_autotune(gct_plan *p, float *vol, uint32_t *best_
         hipEvent_t e0, e1;
         hip_check(hipEventCreate(&e0), "event");
         hip_check(hipEventCreate(&e1), "event");
-        struct Cand { int variant; uint32_t zu; };
-        const Cand cands[5] = {{GCT_BP_LDS, 16u}, {GCT_BP_LDS, 32u}, {GCT_BP_DIRECT, 4u}, {GCT_BP_DIRECT, 8u},
-                               {GCT_BP_DIRECT, 16u}};
+        struct Cand { int variant; uint32_t zu, vb; };
+        const Cand cands[] = {{GCT_BP_LDS, 16u, 2u}, {GCT_BP_LDS, 16u, 3u}, {GCT_BP_LDS, 16u, 4u},
+                              {GCT_BP_LDS, 32u, 1u}, {GCT_BP_LDS, 32u, 2u}, {GCT_BP_DIRECT, 8u, 0u},
+                              {GCT_BP_DIRECT, 16u, 0u}};
         float best = 1e30f;
-        uint32_t bz = p->z_unroll;
-        int bv = p->variant;
+        Cand bc{p->variant, p->z_unroll, p->view_block};
         for (const Cand &c : cands) {
             p->z_unroll = c.zu;
             p->variant = c.variant;
+            p->view_block = c.vb;
             const BpLaunch a = bp_args(*p, vol, 0, p->V, false, true, 0);
             launch_backproject(a, s);   // warm-up
-            float tot = 0.0f;
-            const int reps = 3;
-            for (int r = 0; r < reps; ++r) {
+            float t[5];
+            for (float &ms : t) {
                 hip_check(hipEventRecord(e0, s), "record");
                 launch_backproject(a, s);
                 hip_check(hipEventRecord(e1, s), "record");
                 hip_check(hipEventSynchronize(e1), "sync");
-                float ms = 0.0f;
                 hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
-                tot += ms;
             }
-            if (tot / reps < best) { best = tot / reps; bz = c.zu; bv = c.variant; }
+            std::sort(t, t + 5);
+            if (t[2] < best) { best = t[2]; bc = c; }   // median of five
         }
-        p->variant = bv;
-        p->z_unroll = bz;
+        p->variant = bc.variant;
+        p->z_unroll = bc.zu;
+        p->view_block = bc.vb;
+        const uint32_t bz = bc.zu;
+        const int bv = bc.variant;
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         (void)hipStreamDestroy(s);
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             g_tune[{p->geom.kind, p->img.x.num_samples, p->img.y.num_samples, p->img.dim == 3 ? p->img.z.num_samples : 1u,
-                    p->V, p->nu, p->nv, p->opts.storage, int(p->circular), int(p->setup_f64)}] = Tuning{bz, bv};
+                    p->V, p->nu, p->nv, p->opts.storage, int(p->circular), int(p->setup_f64)}] = Tuning{bz, bv, bc.vb};
         }
         if (best_zu) *best_zu = bz;
         if (best_ms) *best_ms = best;

@@ -86,6 +86,7 @@ struct gct_plan {
     float inv_dgamma = 0.0f;
     float scale = 1.0f;
     uint32_t z_unroll = 16;
+    uint32_t view_block = 0;   // LDS kernel views per staged block; 0: default for z_unroll
     int variant = GCT_BP_LDS;
     hipStream_t stream = nullptr;     // stream-ordered plans (serving) allocate and free on it
     bool ordered = false;

@@ -140,6 +140,7 @@ typedef struct gct_recon_opts {
     uint32_t z_unroll;               /* BP voxels per thread along z; 0 = tuned */
     uint32_t view_chunk;             /* LOW mode views per chunk; 0 = derived */
     int32_t bp_kernel;               /* gct_bp_kernel, default AUTO */
+    uint32_t view_block;             /* LDS BP views staged per barrier: 2..4 (z_unroll 16), 1..2 (32); 0 = tuned */
 } gct_recon_opts;
 
 /* ---- library ----------------------------------------------------------- */
@@ -232,6 +233,7 @@ typedef struct gct_plan_info {
     int32_t circular;                /* 1 when every view keeps the row axis along z */
     int32_t storage;
     int32_t bp_kernel;               /* gct_bp_kernel in use */
+    uint32_t view_block;             /* LDS BP views per staged block in use */
 } gct_plan_info;
 
 /* Builds per-view tables (FP64 on the host, stored FP32), the redundancy
@@ -281,9 +283,11 @@ gct_status gct_batch_execute(gct_batch *batch, const float *const *proj_dev, flo
 
 /* ---- autotuner ----------------------------------------------------------- */
 
-/* Times backprojection for the plan's shape over the candidate z-unroll
- * values and stores the fastest in the plan (and a process-wide cache keyed
- * by shape).  Uses vol_dev as scratch. */
+/* Times backprojection for the plan's shape over the candidate kernels --
+ * LDS-staged with z-unroll 16 (view blocks 2, 3, 4) or 32 (view blocks 1, 2),
+ * direct with z-unroll 8 or 16 -- and stores the fastest in the plan (and a
+ * process-wide cache keyed by shape; gct_plan_get_info reports the choice).
+ * Uses vol_dev as scratch. */
 gct_status gct_plan_autotune(gct_plan *plan, float *vol_dev, uint32_t *best_z_unroll, float *best_ms);
 
 /* ---- serving: continuous batching of reconstruction requests ------------ */
