@@ -100,6 +100,15 @@ class ServerStats(C.Structure):
                 ("peak_inflight", C.c_uint32), ("busy_seconds", C.c_double)]
 
 
+class ArrayMetadata(C.Structure):
+    """ArrayMetadata (metadata.h:23-29): rows x cols per slice, slices per file."""
+    _fields_ = [("num_rows", C.c_uint32), ("num_cols", C.c_uint32), ("num_slices", C.c_uint32), ("type", C.c_int32)]
+
+    def __repr__(self):
+        return (f"ArrayMetadata(rows={self.num_rows}, cols={self.num_cols}, slices={self.num_slices}, "
+                f"type={ELEM_NAMES[self.type]})")
+
+
 class GctError(RuntimeError):
     def __init__(self, status: int, msg: str):
         super().__init__(f"{STATUS.get(status, status)}: {msg}")
@@ -163,6 +172,13 @@ def _declare(L: C.CDLL) -> None:
                                     C.c_double, P]),
         "gct_phantom_project": (st, [C.POINTER(CTGeom), C.c_int, P, C.c_uint32, C.c_double, C.c_uint64, P, C.c_int]),
         "gct_phantom_rasterize": (st, [C.POINTER(ImageGeom), P, C.c_uint32, P]),
+        "gct_file_read_metadata": (st, [C.c_char_p, C.c_int32, C.POINTER(ArrayMetadata), C.POINTER(ArrayMetadata)]),
+        "gct_file_read": (st, [C.c_char_p, C.c_int32, C.c_int, C.POINTER(ArrayMetadata), C.c_int32, P, C.c_uint64,
+                               C.POINTER(C.c_uint64)]),
+        "gct_meta_type_name": (C.c_char_p, [C.c_int32]),
+        "gct_meta_write": (st, [C.c_char_p, C.c_uint32, P, P, P, C.c_int32, P, C.c_uint64]),
+        "gct_den_write": (st, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, P, C.c_uint64]),
+        "gct_raw_write": (st, [C.c_char_p, C.c_int32, P, C.c_uint64]),
         "gct_phantom_project_device": (st, [C.POINTER(CTGeom), C.c_int, P, C.c_uint32, C.c_double, C.c_uint64, P, P]),
         "gct_phantom_rasterize_device": (st, [C.POINTER(ImageGeom), P, C.c_uint32, P, P]),
     }
@@ -425,6 +441,96 @@ def phantom_rasterize_device(img: Image, params: np.ndarray, out, stream=None):
     check(lib().gct_phantom_rasterize_device(C.byref(img.c()), params.ctypes.data, params.shape[0], _ptr(out),
                                              _stream(stream)))
     return out
+
+
+# ---- array files (MetaImage / Dennerlein / raw) ---------------------------------
+
+FILE_AUTO, FILE_META, FILE_DEN, FILE_RAW = range(4)
+_ELEM_DTYPES = [np.uint8, np.int8, np.uint16, np.int16, np.uint32, np.int32, np.uint64, np.int64, np.float32,
+                np.float64]
+ELEM_NAMES = ["uint8", "int8", "uint16", "int16", "uint32", "int32", "uint64", "int64", "float32", "float64"]
+
+
+def elem_type(dtype) -> int:
+    dt = np.dtype(dtype)
+    for i, d in enumerate(_ELEM_DTYPES):
+        if np.dtype(d) == dt:
+            return i
+    raise ValueError(f"unsupported element type {dt}")
+
+
+def meta_type_name(dtype) -> str:
+    return lib().gct_meta_type_name(elem_type(dtype)).decode()
+
+
+def read_metadata(filename, fmt: int = FILE_AUTO, raw_md: ArrayMetadata | None = None) -> ArrayMetadata:
+    md = ArrayMetadata()
+    check(lib().gct_file_read_metadata(os.fsencode(filename), fmt, C.byref(raw_md) if raw_md else None, C.byref(md)))
+    return md
+
+
+def read_array(filename, dtype=np.float32, series: bool = False, fmt: int = FILE_AUTO,
+               raw_md: ArrayMetadata | None = None) -> np.ndarray:
+    """Elements of an array file (or of its series) converted to `dtype`, flat."""
+    t = elem_type(dtype)
+    n = C.c_uint64()
+    raw = C.byref(raw_md) if raw_md else None
+    check(lib().gct_file_read(os.fsencode(filename), fmt, int(series), raw, t, None, 0, C.byref(n)))
+    out = np.empty(n.value, dtype)
+    check(lib().gct_file_read(os.fsencode(filename), fmt, int(series), raw, t, out.ctypes.data, n.value,
+                              C.byref(n)))
+    return out
+
+
+def write_meta(filename, data: np.ndarray, dim_size=None, spacing=None, offset=None):
+    """MetaImage writer; dim_size defaults to the array's shape reversed (x fastest)."""
+    data = np.ascontiguousarray(data)
+    dims = list(dim_size) if dim_size is not None else list(reversed(data.shape))
+    nd = len(dims)
+    sp = np.asarray(spacing if spacing is not None else [0.0] * nd, np.float32)
+    of = np.asarray(offset if offset is not None else [0.0] * nd, np.float32)
+    d = np.asarray(dims, np.uint32)
+    check(lib().gct_meta_write(os.fsencode(filename), nd, d.ctypes.data, sp.ctypes.data, of.ctypes.data,
+                               elem_type(data.dtype), data.ctypes.data, data.size))
+
+
+def write_den(filename, data: np.ndarray, rows: int, cols: int, slices: int):
+    data = np.ascontiguousarray(data)
+    check(lib().gct_den_write(os.fsencode(filename), rows, cols, slices, elem_type(data.dtype), data.ctypes.data,
+                              data.size))
+
+
+def write_raw(filename, data: np.ndarray):
+    data = np.ascontiguousarray(data)
+    check(lib().gct_raw_write(os.fsencode(filename), elem_type(data.dtype), data.ctypes.data, data.size))
+
+
+def read_projections(filename, proj_dim: int, series: bool = False, dtype=np.float32) -> np.ndarray:
+    """ProjectionReader::read (projection_reader.h:18-36): [views][rows][cols]
+    for 2-D detectors, [views][cols] for 1-D ones."""
+    md = read_metadata(filename)
+    pix = read_array(filename, dtype, series)
+    if proj_dim == 1:
+        return pix.reshape(-1, md.num_cols)
+    return pix.reshape(-1, md.num_rows, md.num_cols)
+
+
+def write_image(img: "Image", vol: np.ndarray, filename):
+    """IO::write(ReconImg, filename) (img_writer.h:20-42): .den, or MetaImage
+    with the voxel spacing and the first voxel centre as the offset."""
+    ext = os.path.splitext(str(filename))[1].lower()
+    vol = np.ascontiguousarray(vol)
+    if ext == ".den":
+        write_den(filename, vol, img.ny, img.nx, img.nz if img.dim == 3 else 1)
+    elif ext in (".mha", ".mhd"):
+        n = [img.nx, img.ny] + ([img.nz] if img.dim == 3 else [])
+        w = [img.dx, img.dy] + ([img.dz] if img.dim == 3 else [])
+        mid = list(img.mid)[:img.dim]
+        first = [np.float32(np.float32(m) - (np.float32(k) - np.float32(1.0)) * np.float32(d) / np.float32(2.0))
+                 for m, k, d in zip(mid, n, w)]
+        write_meta(filename, vol, n, w, first)
+    else:
+        raise ValueError("Currently images can be saved to den or meta image file.")
 
 
 def _torch():

@@ -12,7 +12,10 @@
 
 #include <algorithm>
 #include <array>
+#include <cctype>
 #include <cmath>
+#include <cstdint>
+#include <cstdio>
 #include <numbers>
 #include <optional>
 #include <stdexcept>
@@ -457,4 +460,231 @@ inline void reconstruct(Geom::AxialCTGeom<Geom::DetFanLine, Geom::DetFanLine> co
 { Detail::reconstructImpl(cg, proj, vol, opts); }
 
 }  // namespace Recon
+// ---- array files (src/cpp/io): readers, writers, projection and image I/O -------
+namespace IO {
+
+enum class ArrayElemType { T_UINT8, T_INT8, T_UINT16, T_INT16, T_UINT32, T_INT32, T_UINT64, T_INT64, T_FLOAT, T_DOUBLE };
+
+struct ArrayMetadata {
+    unsigned num_rows;
+    unsigned num_cols;
+    unsigned num_slices;   // matrices per file
+    ArrayElemType type;
+};
+
+namespace Detail {
+template <typename T> constexpr int32_t elemType()
+{
+    if constexpr (std::is_same_v<T, std::uint8_t>) return GCT_T_UINT8;
+    else if constexpr (std::is_same_v<T, std::int8_t>) return GCT_T_INT8;
+    else if constexpr (std::is_same_v<T, std::uint16_t>) return GCT_T_UINT16;
+    else if constexpr (std::is_same_v<T, std::int16_t>) return GCT_T_INT16;
+    else if constexpr (std::is_same_v<T, std::uint32_t>) return GCT_T_UINT32;
+    else if constexpr (std::is_same_v<T, std::int32_t>) return GCT_T_INT32;
+    else if constexpr (std::is_same_v<T, std::uint64_t>) return GCT_T_UINT64;
+    else if constexpr (std::is_same_v<T, std::int64_t>) return GCT_T_INT64;
+    else if constexpr (std::is_same_v<T, float>) return GCT_T_FLOAT;
+    else {
+        static_assert(std::is_same_v<T, double>, "unsupported element type");
+        return GCT_T_DOUBLE;
+    }
+}
+inline gct_array_metadata toC(ArrayMetadata const &m)
+{
+    return gct_array_metadata{m.num_rows, m.num_cols, m.num_slices, static_cast<int32_t>(m.type)};
+}
+inline ArrayMetadata fromC(gct_array_metadata const &m)
+{
+    return ArrayMetadata{m.num_rows, m.num_cols, m.num_slices, static_cast<ArrayElemType>(m.type)};
+}
+}  // namespace Detail
+
+// ArrayFileReader (array_file_reader.h:13-72) and its three file kinds.
+class ArrayFileReader {
+public:
+    ArrayMetadata readMetadata() const
+    {
+        gct_array_metadata md{};
+        GCT::Detail::check(gct_file_read_metadata(filename_.c_str(), format_, rawPtr(), &md));
+        return Detail::fromC(md);
+    }
+    // Elements of the file (or of the series), converted by static_cast.
+    template <typename OpType>
+    std::vector<OpType> readData() const
+    {
+        uint64_t n = 0;
+        const int32_t t = Detail::elemType<OpType>();
+        GCT::Detail::check(gct_file_read(filename_.c_str(), format_, series_, rawPtr(), t, nullptr, 0, &n));
+        std::vector<OpType> out(n);
+        GCT::Detail::check(gct_file_read(filename_.c_str(), format_, series_, rawPtr(), t, out.data(), n, &n));
+        return out;
+    }
+
+protected:
+    ArrayFileReader(std::string filename, bool series, int32_t format)
+        : filename_{std::move(filename)}, series_{series ? 1 : 0}, format_{format}
+    {
+        uint64_t n = 0;   // existence and header checks happen on construction, as in the reference
+        if (format_ != GCT_FILE_RAW)
+            GCT::Detail::check(gct_file_read(filename_.c_str(), format_, series_, nullptr, GCT_T_UINT8, nullptr, 0, &n));
+    }
+    virtual const gct_array_metadata *rawPtr() const { return nullptr; }
+    virtual ~ArrayFileReader() = default;
+
+    std::string filename_;
+    int series_;
+    int32_t format_;
+};
+
+// MetaImgFileReader (meta_img_reader.h:12-47): .mha (data after the header) or .mhd + data file.
+class MetaImgFileReader : public ArrayFileReader {
+public:
+    explicit MetaImgFileReader(std::string filename, bool series = false)
+        : ArrayFileReader(std::move(filename), series, GCT_FILE_META) {}
+};
+
+// DenFileReader: three 16-bit sizes (rows, cols, slices), then the data.
+class DenFileReader : public ArrayFileReader {
+public:
+    explicit DenFileReader(std::string filename, bool series = false)
+        : ArrayFileReader(std::move(filename), series, GCT_FILE_DEN) {}
+};
+
+// RawDataFileReader (raw_data_file_reader.h:13-33): headerless, caller's metadata.
+class RawDataFileReader : public ArrayFileReader {
+public:
+    RawDataFileReader(std::string filename, ArrayMetadata md, bool series = false)
+        : ArrayFileReader(std::move(filename), series, GCT_FILE_RAW), md_{Detail::toC(md)}
+    {
+        uint64_t n = 0;
+        GCT::Detail::check(gct_file_read(filename_.c_str(), format_, series_, &md_, GCT_T_UINT8, nullptr, 0, &n));
+    }
+
+protected:
+    const gct_array_metadata *rawPtr() const override { return &md_; }
+
+private:
+    gct_array_metadata md_;
+};
+
+namespace ArrayFileWriter {
+
+template <size_t ImgDim>
+struct MetaImgMetaData {
+    std::array<unsigned, ImgDim> dim_size{};
+    std::array<float, ImgDim> spacing{};
+    std::array<float, ImgDim> offset{};
+};
+
+struct DenMetaData {
+    unsigned short num_rows;
+    unsigned short num_cols;
+    unsigned short num_slices;
+};
+
+namespace Detail::MetaImage {
+template <typename DataElemType> std::string typeStr()
+{
+    return gct_meta_type_name(IO::Detail::elemType<DataElemType>());
+}
+template <size_t ImgDim> size_t numDataElems(std::array<unsigned, ImgDim> const &dim_size)
+{
+    size_t n = 1;
+    for (unsigned d : dim_size) n *= d;
+    return n;
+}
+}  // namespace Detail::MetaImage
+
+// MetaImage writer (meta_img_file_writer.h:101-152).
+template <size_t ImgDim, typename Container>
+void write(MetaImgMetaData<ImgDim> const &md, Container const &data, std::string const &filename)
+{
+    using T = typename Container::value_type;
+    GCT::Detail::check(gct_meta_write(filename.c_str(), static_cast<uint32_t>(ImgDim), md.dim_size.data(),
+                                      md.spacing.data(), md.offset.data(), IO::Detail::elemType<T>(), data.data(),
+                                      data.size()));
+}
+
+// Dennerlein writer (den_file_writer.h:14-60).
+template <typename Container>
+void write(DenMetaData const &md, Container const &data, std::string const &filename)
+{
+    using T = typename Container::value_type;
+    GCT::Detail::check(gct_den_write(filename.c_str(), md.num_rows, md.num_cols, md.num_slices,
+                                     IO::Detail::elemType<T>(), data.data(), data.size()));
+}
+
+// Headerless writer (raw_data_file_writer.h:13-36).
+template <typename Container>
+void write(Container const &data, std::string const &filename)
+{
+    using T = typename Container::value_type;
+    GCT::Detail::check(gct_raw_write(filename.c_str(), IO::Detail::elemType<T>(), data.data(), data.size()));
+}
+
+}  // namespace ArrayFileWriter
+
+// ProjectionReader (projection_reader.h:18-36): 1-D projections are the
+// columns, rows x slices views; 2-D projections are cols x rows, slices views
+// (projection_reader.cpp:5-17).
+template <typename FileReader>
+struct ProjectionReader {
+    template <typename PixType, size_t Dim>
+    static Data::ProjectionStack<PixType, Dim> read(std::string const &filename, bool series = false)
+    {
+        const FileReader reader{filename, series};
+        const ArrayMetadata md = reader.readMetadata();
+        std::vector<PixType> pix = reader.template readData<PixType>();
+        if constexpr (Dim == 1u) {
+            const unsigned views = static_cast<unsigned>(pix.size() / md.num_cols);
+            Data::ProjectionStack<PixType, 1u> p{Data::ProjSize<1u>{md.num_cols}, views};
+            p.setData(std::move(pix));
+            return p;
+        } else {
+            const unsigned views = static_cast<unsigned>(pix.size() / (size_t(md.num_cols) * md.num_rows));
+            Data::ProjectionStack<PixType, 2u> p{Data::ProjSize<2u>{md.num_cols, md.num_rows}, views};
+            p.setData(std::move(pix));
+            return p;
+        }
+    }
+};
+
+// Image writer (img_writer.h:20-42, img_writer.cpp:5-62): .den, or MetaImage
+// with the voxel spacing and the first voxel's centre as the offset.
+template <typename PixType, size_t Dim>
+void write(Data::ReconImg<PixType, Dim> const &img, std::string const &filename)
+{
+    std::string ext;
+    if (const auto dot = filename.find_last_of('.'); dot != std::string::npos) ext = filename.substr(dot);
+    std::transform(ext.begin(), ext.end(), ext.begin(), [](unsigned char c) { return char(std::tolower(c)); });
+    const auto &g = img.geom();
+    if (ext == ".den") {
+        ArrayFileWriter::DenMetaData md{static_cast<unsigned short>(g.y.num_samples),
+                                        static_cast<unsigned short>(g.x.num_samples), 1u};
+        if constexpr (Dim == 3u) md.num_slices = static_cast<unsigned short>(g.z.num_samples);
+        ArrayFileWriter::write(md, img.elements(), filename);
+    } else if (ext == ".mha" || ext == ".mhd") {
+        auto first = [](Geom::EqDistSampler const &s, float mid) {
+            return mid - (static_cast<float>(s.num_samples) - 1.0f) * s.sampling_width.val / 2.0f;
+        };
+        ArrayFileWriter::MetaImgMetaData<Dim> md;
+        md.dim_size[0] = g.x.num_samples;
+        md.dim_size[1] = g.y.num_samples;
+        md.spacing[0] = g.x.sampling_width.val;
+        md.spacing[1] = g.y.sampling_width.val;
+        md.offset[0] = first(g.x, g.mid_point.x.val);
+        md.offset[1] = first(g.y, g.mid_point.y.val);
+        if constexpr (Dim == 3u) {
+            md.dim_size[2] = g.z.num_samples;
+            md.spacing[2] = g.z.sampling_width.val;
+            md.offset[2] = first(g.z, g.mid_point.z.val);
+        }
+        ArrayFileWriter::write(md, img.elements(), filename);
+    } else {
+        std::fprintf(stderr, "Currently images can be saved to den or meta image file.\n");
+    }
+}
+
+}  // namespace IO
+
 }  // namespace GCT

@@ -349,6 +349,55 @@ gct_status gct_phantom_project_device(const gct_ct_geom *geom, int dim, const do
 gct_status gct_phantom_rasterize_device(const gct_image_geom *img, const double *params, uint32_t count,
                                         float *vol_dev, void *stream);
 
+/* ---- array files: MetaImage, Dennerlein, raw (the reference's src/cpp/io) -- */
+
+/* ArrayElemType (metadata.h:9-21). */
+typedef enum gct_elem_type {
+    GCT_T_UINT8 = 0, GCT_T_INT8, GCT_T_UINT16, GCT_T_INT16, GCT_T_UINT32, GCT_T_INT32,
+    GCT_T_UINT64, GCT_T_INT64, GCT_T_FLOAT, GCT_T_DOUBLE
+} gct_elem_type;
+
+typedef enum gct_file_format {
+    GCT_FILE_AUTO = 0,   /* by extension: .mha/.mhd MetaImage, .den Dennerlein, else raw */
+    GCT_FILE_META = 1,   /* MetaImgFileReader (meta_img_reader.cpp:10-165) */
+    GCT_FILE_DEN = 2,    /* DenFileReader (den_file_reader.cpp:7-61) */
+    GCT_FILE_RAW = 3     /* RawDataFileReader (raw_data_file_reader.h:13-33): caller's metadata */
+} gct_file_format;
+
+/* ArrayMetadata (metadata.h:23-29): rows x cols per slice, slices per file. */
+typedef struct gct_array_metadata {
+    uint32_t num_rows;
+    uint32_t num_cols;
+    uint32_t num_slices;
+    int32_t type;                    /* gct_elem_type */
+} gct_array_metadata;
+
+/* ArrayFileReader::readMetadata.  raw_md: the metadata of a raw file (else NULL). */
+gct_status gct_file_read_metadata(const char *filename, int32_t format, const gct_array_metadata *raw_md,
+                                  gct_array_metadata *md);
+/* ArrayFileReader::readData<T> (array_file_reader.h:66-72): the elements of the
+ * file -- or, with series != 0, of every file in its directory tree with the
+ * same extension and size, in name order (getAllFiles, file_utils.cpp:7-28) --
+ * converted to out_type by static_cast.  *count receives the element count;
+ * out == NULL only queries it.  GCT_ERR_RUNTIME for the reference's
+ * runtime_error cases (missing file or data file, size mismatch, bad
+ * ElementType). */
+gct_status gct_file_read(const char *filename, int32_t format, int series, const gct_array_metadata *raw_md,
+                         int32_t out_type, void *out, uint64_t capacity, uint64_t *count);
+/* "MET_FLOAT" etc. (MetaImage::typeStr, meta_img_file_writer.cpp:11-69); NULL if unknown. */
+const char *gct_meta_type_name(int32_t type);
+/* ArrayFileWriter::write for MetaImage (meta_img_file_writer.h:101-152): the
+ * reference's header lines; .mha holds the data after the header, any other
+ * extension writes the data to <stem>.raw beside it.  num_elems must equal
+ * the product of dim_size. */
+gct_status gct_meta_write(const char *filename, uint32_t ndims, const uint32_t *dim_size, const float *spacing,
+                          const float *offset, int32_t type, const void *data, uint64_t num_elems);
+/* ArrayFileWriter::write for Dennerlein files (den_file_writer.h:14-60). */
+gct_status gct_den_write(const char *filename, uint32_t num_rows, uint32_t num_cols, uint32_t num_slices,
+                         int32_t type, const void *data, uint64_t num_elems);
+/* ArrayFileWriter::write for headerless files (raw_data_file_writer.h:13-36). */
+gct_status gct_raw_write(const char *filename, int32_t type, const void *data, uint64_t num_elems);
+
 #ifdef __cplusplus
 }
 #endif

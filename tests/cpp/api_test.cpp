@@ -11,7 +11,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
+#include <unistd.h>
 #include <numbers>
 #include <string>
 
@@ -128,6 +130,46 @@ int main(int argc, char **argv)
             threw = true;
         }
         CHECK(threw);
+    }
+
+    // array files through GCT::IO (meta_img_file_writer_tests.cpp, mhd_file_reader_tests.cpp):
+    // the 3 x 2 x 4 iota fixture written as .mha and .mhd and read back, a series, a DEN image
+    {
+        namespace fs = std::filesystem;
+        const fs::path dir = fs::temp_directory_path() / ("gct_io_" + std::to_string(::getpid()));
+        fs::create_directories(dir);
+        std::vector<float> data(24), data2(24);
+        for (int i = 0; i < 24; ++i) { data[i] = float(i); data2[i] = float(24 + i); }
+        IO::ArrayFileWriter::MetaImgMetaData<3u> md;
+        md.dim_size = {3u, 2u, 4u};
+        for (const char *ext : {".mha", ".mhd"}) {
+            const std::string f1 = (dir / (std::string("01") + ext)).string();
+            const std::string f2 = (dir / (std::string("02") + ext)).string();
+            IO::ArrayFileWriter::write(md, data, f1);
+            IO::ArrayFileWriter::write(md, data2, f2);
+            const IO::MetaImgFileReader r{f1};
+            const auto m = r.readMetadata();
+            CHECK(m.num_rows == 2u && m.num_cols == 3u && m.num_slices == 4u && m.type == IO::ArrayElemType::T_FLOAT);
+            CHECK(r.readData<float>() == data);
+            const auto all = IO::MetaImgFileReader{f1, true}.readData<float>();
+            CHECK(all.size() == 48u && all[0] == 0.0f && all[47] == 47.0f);
+            const auto proj = IO::ProjectionReader<IO::MetaImgFileReader>::read<float, 2u>(f1);
+            CHECK(proj.projSize().num_cols == 3u && proj.projSize().num_rows == 2u && proj.numProj() == 4u);
+        }
+        CHECK(IO::ArrayFileWriter::Detail::MetaImage::typeStr<std::uint16_t>() == "MET_USHORT");
+        CHECK(IO::ArrayFileWriter::Detail::MetaImage::numDataElems(std::array<unsigned, 3UL>{2u, 3u, 4u}) == 24UL);
+        bool threw = false;
+        try {
+            IO::ArrayFileWriter::write(md, std::vector<float>(1), (dir / "invalid_dir" / "raw.mha").string());
+        } catch (std::runtime_error const &) {
+            threw = true;
+        }
+        CHECK(threw);
+        Data::ReconImg<float, 2u> im{ReconImgGeom<2u>{{3u, 0.5_mm}, {2u, 0.5_mm}, {}}};
+        IO::write(im, (dir / "img.den").string());
+        const auto dm = IO::DenFileReader{(dir / "img.den").string()}.readMetadata();
+        CHECK(dm.num_rows == 2u && dm.num_cols == 3u && dm.num_slices == 1u && dm.type == IO::ArrayElemType::T_FLOAT);
+        fs::remove_all(dir);
     }
 
     if (argc >= 3 && std::strcmp(argv[1], "--gpu") == 0) {
