@@ -55,32 +55,35 @@ __global__ void redundancy_kernel(const float *fan, const float *beta, uint32_t 
 // reads x[i + j T], multiplies by exp(-2 pi i j k / (R p)), k = i mod p,
 // applies a length-R DFT and writes y[(i - k) R + k + j p].  Each thread keeps
 // its butterflies in registers across one barrier, so a single LDS buffer of
-// L complex values suffices.
+// L complex values suffices.  Twiddles are read from an LDS copy of the table
+// exp(-2 pi i m / L), m < L.
 
 __device__ inline float2 cmul(float2 a, float2 b)
 {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+__device__ inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ inline float2 mul_mi(float2 a) { return make_float2(a.y, -a.x); }   // a * (-i)
+
 template <int R> __device__ inline void dft(float2 *u);
 
 template <> __device__ inline void dft<2>(float2 *u)
 {
-    float2 a = u[0], b = u[1];
-    u[0] = make_float2(a.x + b.x, a.y + b.y);
-    u[1] = make_float2(a.x - b.x, a.y - b.y);
+    const float2 a = u[0], b = u[1];
+    u[0] = cadd(a, b);
+    u[1] = csub(a, b);
 }
 
 template <> __device__ inline void dft<4>(float2 *u)
 {
-    float2 a = make_float2(u[0].x + u[2].x, u[0].y + u[2].y);
-    float2 b = make_float2(u[0].x - u[2].x, u[0].y - u[2].y);
-    float2 c = make_float2(u[1].x + u[3].x, u[1].y + u[3].y);
-    float2 d = make_float2(u[1].x - u[3].x, u[1].y - u[3].y);
-    u[0] = make_float2(a.x + c.x, a.y + c.y);
-    u[2] = make_float2(a.x - c.x, a.y - c.y);
-    u[1] = make_float2(b.x + d.y, b.y - d.x);   // b - i d
-    u[3] = make_float2(b.x - d.y, b.y + d.x);   // b + i d
+    const float2 a = cadd(u[0], u[2]), b = csub(u[0], u[2]);
+    const float2 c = cadd(u[1], u[3]), d = csub(u[1], u[3]);
+    u[0] = cadd(a, c);
+    u[2] = csub(a, c);
+    u[1] = cadd(b, mul_mi(d));   // b - i d
+    u[3] = csub(b, mul_mi(d));   // b + i d
 }
 
 template <> __device__ inline void dft<8>(float2 *u)
@@ -91,21 +94,53 @@ template <> __device__ inline void dft<8>(float2 *u)
     dft<4>(o);
     const float r = 0.70710678118654752f;
     // o[m] *= W8^m, W8 = exp(-i pi / 4)
-    float2 o1 = make_float2(r * (o[1].x + o[1].y), r * (o[1].y - o[1].x));
-    float2 o2 = make_float2(o[2].y, -o[2].x);
-    float2 o3 = make_float2(r * (o[3].y - o[3].x), -r * (o[3].x + o[3].y));
-    u[0] = make_float2(e[0].x + o[0].x, e[0].y + o[0].y);
-    u[4] = make_float2(e[0].x - o[0].x, e[0].y - o[0].y);
-    u[1] = make_float2(e[1].x + o1.x, e[1].y + o1.y);
-    u[5] = make_float2(e[1].x - o1.x, e[1].y - o1.y);
-    u[2] = make_float2(e[2].x + o2.x, e[2].y + o2.y);
-    u[6] = make_float2(e[2].x - o2.x, e[2].y - o2.y);
-    u[3] = make_float2(e[3].x + o3.x, e[3].y + o3.y);
-    u[7] = make_float2(e[3].x - o3.x, e[3].y - o3.y);
+    const float2 o1 = make_float2(r * (o[1].x + o[1].y), r * (o[1].y - o[1].x));
+    const float2 o2 = mul_mi(o[2]);
+    const float2 o3 = make_float2(r * (o[3].y - o[3].x), -r * (o[3].x + o[3].y));
+    u[0] = cadd(e[0], o[0]);
+    u[4] = csub(e[0], o[0]);
+    u[1] = cadd(e[1], o1);
+    u[5] = csub(e[1], o1);
+    u[2] = cadd(e[2], o2);
+    u[6] = csub(e[2], o2);
+    u[3] = cadd(e[3], o3);
+    u[7] = csub(e[3], o3);
+}
+
+// 16-point DFT as 4 x 4: length-4 DFTs over the stride-4 subsequences, the
+// twiddles W16^(n1 k2), then length-4 DFTs across (output index k2 + 4 k1).
+template <> __device__ inline void dft<16>(float2 *u)
+{
+    float2 t[4][4];
+#pragma unroll
+    for (int n1 = 0; n1 < 4; ++n1) {
+        float2 v[4] = {u[n1], u[n1 + 4], u[n1 + 8], u[n1 + 12]};
+        dft<4>(v);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) t[n1][k2] = v[k2];
+    }
+    const float c1 = 0.92387953251128674f, s1 = 0.38268343236508977f, r = 0.70710678118654752f;
+    // W16^m = exp(-2 pi i m / 16) for the products n1 k2 in {1, 2, 3, 4, 6, 9}
+    t[1][1] = cmul(t[1][1], make_float2(c1, -s1));
+    t[1][2] = make_float2(r * (t[1][2].x + t[1][2].y), r * (t[1][2].y - t[1][2].x));
+    t[1][3] = cmul(t[1][3], make_float2(s1, -c1));
+    t[2][1] = make_float2(r * (t[2][1].x + t[2][1].y), r * (t[2][1].y - t[2][1].x));
+    t[2][2] = mul_mi(t[2][2]);
+    t[2][3] = make_float2(r * (t[2][3].y - t[2][3].x), -r * (t[2][3].x + t[2][3].y));
+    t[3][1] = cmul(t[3][1], make_float2(s1, -c1));
+    t[3][2] = make_float2(r * (t[3][2].y - t[3][2].x), -r * (t[3][2].x + t[3][2].y));
+    t[3][3] = cmul(t[3][3], make_float2(-c1, s1));
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) {
+        float2 v[4] = {t[0][k2], t[1][k2], t[2][k2], t[3][k2]};
+        dft<4>(v);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) u[k2 + 4 * k1] = v[k1];
+    }
 }
 
 template <uint32_t L, uint32_t R, uint32_t P, uint32_t NT>
-__device__ inline void stockham_pass(float2 *buf, const float2 *__restrict__ tw, uint32_t tid)
+__device__ inline void stockham_pass(float2 *buf, const float2 *tw, uint32_t tid)
 {
     constexpr uint32_t T = L / R;
     constexpr uint32_t B = (T + NT - 1) / NT;
@@ -138,12 +173,18 @@ __device__ inline void stockham_pass(float2 *buf, const float2 *__restrict__ tw,
     __syncthreads();
 }
 
+// Radix order: 16 while at least 16 remain (8 then 4 for 32), then the rest.
+template <uint32_t rem>
+constexpr uint32_t next_radix()
+{
+    return rem == 32u ? 8u : (rem >= 16u ? 16u : rem);
+}
+
 template <uint32_t L, uint32_t P, uint32_t NT>
-__device__ inline void fft_from(float2 *buf, const float2 *__restrict__ tw, uint32_t tid)
+__device__ inline void fft_from(float2 *buf, const float2 *tw, uint32_t tid)
 {
     if constexpr (P < L) {
-        constexpr uint32_t rem = L / P;
-        constexpr uint32_t R = (rem == 16u) ? 4u : (rem >= 8u ? 8u : rem);
+        constexpr uint32_t R = next_radix<L / P>();
         stockham_pass<L, R, P, NT>(buf, tw, tid);
         fft_from<L, P * R, NT>(buf, tw, tid);
     }
@@ -151,72 +192,105 @@ __device__ inline void fft_from(float2 *buf, const float2 *__restrict__ tw, uint
 
 // ---- fused pre-weighting + ramp filter ------------------------------------
 
+// cos_weighting.cuh:16-153 with PCSToDCS = (pix - centre) * width; D / sqrt(q)
+// is evaluated as D * rsq(q) (1 ulp).
 template <int KIND>
 __device__ inline float cos_weight(const FiltRec &f, float du, float dv, uint32_t c, uint32_t r)
 {
-    // cos_weighting.cuh:16-153; PCSToDCS = (pix - centre) * width
     const float u = (static_cast<float>(c) - f.cu) * du;
     const float v = (static_cast<float>(r) - f.cv) * dv;
+    const float d2 = f.sdd * f.sdd;
     if constexpr (KIND == GCT_FAN_ARC_1D) return cosf(u);
-    if constexpr (KIND == GCT_FAN_LINE_1D) return f.sdd / sqrtf(u * u + f.sdd * f.sdd);
-    if constexpr (KIND == GCT_CONE_PARALLEL) return f.sdd / sqrtf(v * v + f.sdd * f.sdd);
-    if constexpr (KIND == GCT_CYLINDRICAL) return cosf(u) * (f.sdd / sqrtf(v * v + f.sdd * f.sdd));
-    if constexpr (KIND == GCT_FLAT) return f.sdd / sqrtf(u * u + v * v + f.sdd * f.sdd);
+    if constexpr (KIND == GCT_FAN_LINE_1D) return f.sdd * __builtin_amdgcn_rsqf(fmaf(u, u, d2));
+    if constexpr (KIND == GCT_CONE_PARALLEL) return f.sdd * __builtin_amdgcn_rsqf(fmaf(v, v, d2));
+    if constexpr (KIND == GCT_CYLINDRICAL) return cosf(u) * (f.sdd * __builtin_amdgcn_rsqf(fmaf(v, v, d2)));
+    if constexpr (KIND == GCT_FLAT) return f.sdd * __builtin_amdgcn_rsqf(fmaf(u, u, fmaf(v, v, d2)));
     return 1.0f;
 }
 
-constexpr uint32_t kFiltThreads = 64;
+// Threads per row pair: L below 64, one wavefront up to L = 1024, then L / 16
+// (one radix-16 butterfly per thread and pass); a 256-thread workgroup holds
+// 256 / NT row pairs, which share the LDS twiddle table.
+template <uint32_t L>
+constexpr uint32_t filt_nt()
+{
+    return L < 64u ? L : (L <= 1024u ? 64u : L / 16u);
+}
+constexpr uint32_t kFiltBlock = 256;
 
-// One wavefront filters two detector rows as one complex signal z = a + i b:
-// IFFT(FFT(z) H) = (a * h) + i (b * h) because the kernel h is real.  H is the
-// FFT of the N-tap kernel circularly shifted by floor(N/2) and scaled by 1/L,
-// so output sample c is the reference's y[c + floor(N/2)] / L
+// One group of NT threads filters two detector rows as one complex signal
+// z = a + i b: IFFT(FFT(z) H) = (a * h) + i (b * h) because the kernel h is
+// real.  H is the FFT of the N-tap kernel circularly shifted by floor(N/2)
+// and scaled by 1/L, so output sample c is the reference's y[c + floor(N/2)] / L
 // (ramp_filtering_fft.cu:45-62).  The inverse FFT is conj(FFT(conj(.))).
 template <int KIND, int S, uint32_t L>
-__global__ __launch_bounds__(kFiltThreads) void filter_kernel(FilterLaunch a)
+__global__ __launch_bounds__(kFiltBlock) void filter_kernel(FilterLaunch a)
 {
-    __shared__ float2 buf[L];
+    constexpr uint32_t NT = filt_nt<L>(), G = kFiltBlock / NT;
+    __shared__ float2 tw[L];
+    __shared__ float2 bufs[G][L];
     using St = Stor<S>;
-    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = threadIdx.x; i < L; i += kFiltBlock) tw[i] = a.tw[i];
+    const uint32_t grp = threadIdx.x / NT, tid = threadIdx.x % NT;
+    float2 *buf = bufs[grp];
     const uint32_t nv = a.nv, nu = a.nu;
     const uint32_t nrows = a.num_views * nv;
-    const uint32_t r0 = 2u * blockIdx.x, r1 = r0 + 1u;
-    const bool has1 = r1 < nrows;
-    const uint32_t v0 = a.first_view + r0 / nv, j0 = r0 % nv;
-    const uint32_t v1 = a.first_view + (has1 ? r1 : r0) / nv, j1 = (has1 ? r1 : r0) % nv;
+    const uint32_t r0 = 2u * (blockIdx.x * G + grp), r1 = r0 + 1u;
+    const bool has0 = r0 < nrows, has1 = r1 < nrows;
+    const uint32_t q0 = has0 ? r0 : 0u, q1 = has1 ? r1 : q0;
+    const uint32_t v0 = a.first_view + q0 / nv, j0 = q0 % nv;
+    const uint32_t v1 = a.first_view + q1 / nv, j1 = q1 % nv;
     const float *pa = a.proj + (static_cast<size_t>(v0 - a.proj_view_offset) * nv + j0) * nu;
     const float *pb = a.proj + (static_cast<size_t>(v1 - a.proj_view_offset) * nv + j1) * nu;
     const FiltRec f0 = a.frec[v0], f1 = a.frec[v1];
     const float *red0 = a.red + static_cast<size_t>(v0) * nu;
     const float *red1 = a.red + static_cast<size_t>(v1) * nu;
 
-    for (uint32_t c = tid; c < L; c += kFiltThreads) {
-        float x = 0.0f, y = 0.0f;
-        if (c < nu) {
-            // fdk_axial.cuh:171-173: p *= cosine weight, then p *= redundancy weight
-            x = pa[c] * cos_weight<KIND>(f0, a.du, a.dv, c, j0);
-            x = x * red0[c];
-            if (has1) {
-                y = pb[c] * cos_weight<KIND>(f1, a.du, a.dv, c, j1);
-                y = y * red1[c];
-            }
+    // Every global load of a stage is issued before its first use (the loops
+    // are unrolled over the L / NT samples of each thread).
+    constexpr uint32_t KI = L / NT;
+    {
+        float xa[KI], xb[KI], wa[KI], wb[KI];
+#pragma unroll
+        for (uint32_t k = 0; k < KI; ++k) {
+            const uint32_t c = tid + k * NT;
+            const bool in = c < nu && has0;
+            xa[k] = in ? pa[c] : 0.0f;
+            wa[k] = in ? red0[c] : 0.0f;
+            xb[k] = (in && has1) ? pb[c] : 0.0f;
+            wb[k] = (in && has1) ? red1[c] : 0.0f;
         }
-        buf[c] = make_float2(x, y);
+#pragma unroll
+        for (uint32_t k = 0; k < KI; ++k) {
+            const uint32_t c = tid + k * NT;
+            // fdk_axial.cuh:171-173: p *= cosine weight, then p *= redundancy weight
+            const float x = (xa[k] * cos_weight<KIND>(f0, a.du, a.dv, c, j0)) * wa[k];
+            const float y = (xb[k] * cos_weight<KIND>(f1, a.du, a.dv, c, j1)) * wb[k];
+            buf[c] = make_float2(x, y);
+        }
     }
     __syncthreads();
-    fft_from<L, 1, kFiltThreads>(buf, a.tw, tid);
-    for (uint32_t c = tid; c < L; c += kFiltThreads) {
-        const float2 z = cmul(buf[c], a.H[c]);
-        buf[c] = make_float2(z.x, -z.y);
+    fft_from<L, 1, NT>(buf, tw, tid);
+    {
+        float2 h[KI];
+#pragma unroll
+        for (uint32_t k = 0; k < KI; ++k) h[k] = a.H[tid + k * NT];
+#pragma unroll
+        for (uint32_t k = 0; k < KI; ++k) {
+            const uint32_t c = tid + k * NT;
+            const float2 z = cmul(buf[c], h[k]);
+            buf[c] = make_float2(z.x, -z.y);
+        }
     }
     __syncthreads();
-    fft_from<L, 1, kFiltThreads>(buf, a.tw, tid);
+    fft_from<L, 1, NT>(buf, tw, tid);
+    if (!has0) return;
 
     typename St::T *out = static_cast<typename St::T *>(a.filt);
     const size_t view_elems = static_cast<size_t>(a.rows_pv) * a.pitch;
     typename St::T *oa = out + static_cast<size_t>(v0) * view_elems + static_cast<size_t>(j0 + 1u) * a.pitch + 1u;
     typename St::T *ob = out + static_cast<size_t>(v1) * view_elems + static_cast<size_t>(j1 + 1u) * a.pitch + 1u;
-    for (uint32_t c = tid; c < nu; c += kFiltThreads) {
+    for (uint32_t c = tid; c < nu; c += NT) {
         const float2 z = buf[c];
         oa[c] = St::cv(z.x);
         if (has1) ob[c] = St::cv(-z.y);
@@ -487,19 +561,24 @@ template <int KIND, int S>
 void launch_filter_L(const FilterLaunch &a, hipStream_t s)
 {
     const uint32_t nrows = a.num_views * (is_1d(KIND) ? 1u : a.nv);
-    const uint32_t blocks = (nrows + 1u) / 2u;
-    if (blocks == 0) return;
+    const uint32_t pairs = (nrows + 1u) / 2u;
+    if (pairs == 0) return;
+    // workgroups of 256 threads, 256 / filt_nt<L>() row pairs each
+    auto go = [&](auto kern, uint32_t nt) {
+        const uint32_t g = kFiltBlock / nt;
+        hipLaunchKernelGGL(kern, dim3((pairs + g - 1u) / g), dim3(kFiltBlock), 0, s, a);
+    };
     switch (a.L) {
-    case 8: hipLaunchKernelGGL((filter_kernel<KIND, S, 8>), dim3(blocks), dim3(kFiltThreads), 0, s, a); break;
-    case 16: hipLaunchKernelGGL((filter_kernel<KIND, S, 16>), dim3(blocks), dim3(kFiltThreads), 0, s, a); break;
-    case 32: hipLaunchKernelGGL((filter_kernel<KIND, S, 32>), dim3(blocks), dim3(kFiltThreads), 0, s, a); break;
-    case 64: hipLaunchKernelGGL((filter_kernel<KIND, S, 64>), dim3(blocks), dim3(kFiltThreads), 0, s, a); break;
-    case 128: hipLaunchKernelGGL((filter_kernel<KIND, S, 128>), dim3(blocks), dim3(kFiltThreads), 0, s, a); break;
-    case 256: hipLaunchKernelGGL((filter_kernel<KIND, S, 256>), dim3(blocks), dim3(kFiltThreads), 0, s, a); break;
-    case 512: hipLaunchKernelGGL((filter_kernel<KIND, S, 512>), dim3(blocks), dim3(kFiltThreads), 0, s, a); break;
-    case 1024: hipLaunchKernelGGL((filter_kernel<KIND, S, 1024>), dim3(blocks), dim3(kFiltThreads), 0, s, a); break;
-    case 2048: hipLaunchKernelGGL((filter_kernel<KIND, S, 2048>), dim3(blocks), dim3(kFiltThreads), 0, s, a); break;
-    case 4096: hipLaunchKernelGGL((filter_kernel<KIND, S, 4096>), dim3(blocks), dim3(kFiltThreads), 0, s, a); break;
+    case 8: go(filter_kernel<KIND, S, 8>, filt_nt<8>()); break;
+    case 16: go(filter_kernel<KIND, S, 16>, filt_nt<16>()); break;
+    case 32: go(filter_kernel<KIND, S, 32>, filt_nt<32>()); break;
+    case 64: go(filter_kernel<KIND, S, 64>, filt_nt<64>()); break;
+    case 128: go(filter_kernel<KIND, S, 128>, filt_nt<128>()); break;
+    case 256: go(filter_kernel<KIND, S, 256>, filt_nt<256>()); break;
+    case 512: go(filter_kernel<KIND, S, 512>, filt_nt<512>()); break;
+    case 1024: go(filter_kernel<KIND, S, 1024>, filt_nt<1024>()); break;
+    case 2048: go(filter_kernel<KIND, S, 2048>, filt_nt<2048>()); break;
+    case 4096: go(filter_kernel<KIND, S, 4096>, filt_nt<4096>()); break;
     default: fail(GCT_ERR_UNSUPPORTED, "FFT length outside 8..4096");
     }
 }
