@@ -172,6 +172,8 @@ def _declare(L: C.CDLL) -> None:
                                     C.c_double, P]),
         "gct_phantom_project": (st, [C.POINTER(CTGeom), C.c_int, P, C.c_uint32, C.c_double, C.c_uint64, P, C.c_int]),
         "gct_phantom_rasterize": (st, [C.POINTER(ImageGeom), P, C.c_uint32, P]),
+        "gct_spiral_windows": (st, [C.POINTER(CTGeom), C.POINTER(ImageGeom), P]),
+        "gct_reconstruct_spiral": (st, [C.POINTER(CTGeom), P, C.POINTER(ImageGeom), P, C.POINTER(ReconOpts)]),
         "gct_file_read_metadata": (st, [C.c_char_p, C.c_int32, C.POINTER(ArrayMetadata), C.POINTER(ArrayMetadata)]),
         "gct_file_read": (st, [C.c_char_p, C.c_int32, C.c_int, C.POINTER(ArrayMetadata), C.c_int32, P, C.c_uint64,
                                C.POINTER(C.c_uint64)]),
@@ -242,11 +244,12 @@ class Scan:
 def circular_scan(kind: int, nu: int, du: float, num_proj: int, ang_incr: float, init_ang: float = 0.0,
                   nv: int = 1, dv: float = 1.0, det_row_center: float | None = None,
                   det_col_center: float | None = None, src_det_dist: float = 0.0, src_iso_dist: float = 0.0,
-                  direction: int = 1) -> Scan:
-    """ProjGeomFactory<...>::create for every view of an ideal circular scan."""
+                  direction: int = 1, hel_pitch: float = 0.0) -> Scan:
+    """ProjGeomFactory<...>::create for every view of an ideal circular (or,
+    with hel_pitch != 0, helical: source z = -hel_pitch * angle) scan."""
     p = CircularParams()
     p.kind, p.dir = kind, direction
-    p.init_ang, p.ang_incr, p.num_proj, p.hel_pitch = init_ang, ang_incr, num_proj, 0.0
+    p.init_ang, p.ang_incr, p.num_proj, p.hel_pitch = init_ang, ang_incr, num_proj, hel_pitch
     p.det_row_center = (nu - 1) / 2.0 if det_row_center is None else det_row_center
     p.det_col_center = (nv - 1) / 2.0 if det_col_center is None else det_col_center
     p.src_det_dist, p.src_iso_dist = src_det_dist, src_iso_dist
@@ -294,6 +297,25 @@ def rotation_geom(scan: Scan):
     rel = np.zeros(scan.num_proj, np.float32)
     check(lib().gct_rotation_geom_compute(C.byref(scan.c()), C.byref(r), rot.ctypes.data, rel.ctypes.data))
     return r, rot, rel
+
+
+def spiral_windows(scan: Scan, img: Image) -> np.ndarray:
+    """[first, last) views of every slice of a spiral reconstruction ([nz][2])."""
+    win = np.zeros((img.nz, 2), np.uint32)
+    check(lib().gct_spiral_windows(C.byref(scan.c()), C.byref(img.c()), win.ctypes.data))
+    return win
+
+
+def reconstruct_spiral(scan: Scan, proj: np.ndarray, img: Image, **kw) -> np.ndarray:
+    """Spiral FDK, host to host (Recon::CUDA::reconstruct for SpiralCTGeom)."""
+    proj = np.ascontiguousarray(proj, dtype=np.float32)
+    if proj.size != scan.num_proj * scan.nv * scan.nu:
+        raise ValueError("projection stack does not match the scan")
+    vol = np.zeros(img.shape, np.float32)
+    o = opts(**kw)
+    check(lib().gct_reconstruct_spiral(C.byref(scan.c()), proj.ctypes.data, C.byref(img.c()), vol.ctypes.data,
+                                       C.byref(o)))
+    return vol
 
 
 def ramp_kernel(scan: Scan, cutoff: float = 1.0, window: int = WINDOW_NONE) -> np.ndarray:

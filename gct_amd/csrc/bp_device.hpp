@@ -358,7 +358,10 @@ __device__ inline void tile_boxes(const BpLaunch &a, uint32_t bx, uint32_t by, u
 // the 256-thread workgroup calls it (it holds workgroup barriers).  `tile`
 // and `boxes` are the workgroup's LDS staging area of VB * TR * kTP pairs and
 // VB boxes.
-template <int KIND, int S, bool F64, int ZU, int TR, int VB>
+// WIN: spiral scans -- slice z only takes the views [zwin[2z], zwin[2z+1]);
+// the tile's view loop covers the union of its slices' windows, and the
+// per-slice test is on wave-uniform values (scalar branches).
+template <int KIND, int S, bool F64, int ZU, int TR, int VB, bool WIN = false>
 __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint32_t bz, uint32_t tid, float2 *tile,
                               Box *boxes)
 {
@@ -378,9 +381,25 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
     const float nuf = static_cast<float>(a.nu), nvf = static_cast<float>(a.nv);
     const size_t view_elems = static_cast<size_t>(a.rows_pv) * a.pitch;
     const T *base = static_cast<const T *>(a.filt);
-    const uint32_t vend = a.first_view + a.num_views;
+    uint32_t vfirst = a.first_view, vend = a.first_view + a.num_views;
+    uint32_t wl[WIN ? ZU : 1], wh[WIN ? ZU : 1];
+    if constexpr (WIN) {
+        uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+#pragma unroll
+        for (int k = 0; k < ZU; ++k) {
+            const uint32_t z = min(iz0 + static_cast<uint32_t>(k), a.nz - 1u);
+            wl[k] = a.zwin[2u * z];
+            wh[k] = a.zwin[2u * z + 1u];
+            if (static_cast<uint32_t>(k) < zn && wl[k] < wh[k]) {
+                lo = min(lo, wl[k]);
+                hi = max(hi, wh[k]);
+            }
+        }
+        vfirst = max(vfirst, lo);
+        vend = min(vend, hi);
+    }
 
-    for (uint32_t vb0 = a.first_view; vb0 < vend; vb0 += VB) {
+    for (uint32_t vb0 = vfirst; vb0 < vend; vb0 += VB) {
         const uint32_t nvb = min(static_cast<uint32_t>(VB), vend - vb0);
         tile_boxes<KIND, ZU>(a, bx, by, bz, vb0, nvb, TR, boxes, tid);
         __syncthreads();
@@ -435,6 +454,9 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                     const float rlo = static_cast<float>(b.rlo), rhi = static_cast<float>(b.rlo + b.nrows - 2);
 #pragma unroll
                     for (int k = 0; k < ZU; ++k) {
+                        if constexpr (WIN) {
+                            if (v < wl[k] || v >= wh[k]) continue;
+                        }
                         const float zk = fz + static_cast<float>(k);
                         const float row = __builtin_amdgcn_fmed3f(fmaf(zk, s.drow, s.row0), rlo, rhi);
                         const float rf = floorf(row);
@@ -457,6 +479,9 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                     const Sample cs = col_sample(s.col, nuf);
 #pragma unroll
                     for (int k = 0; k < ZU; ++k) {
+                        if constexpr (WIN) {
+                            if (v < wl[k] || v >= wh[k]) continue;
+                        }
                         const float zk = fz + static_cast<float>(k);
                         const float row = fmaf(zk, s.drow, s.row0);
                         const float val = bilinear<S>(p, a.pitch, cs, row, nvf);
@@ -491,7 +516,7 @@ constexpr int lds_waves_per_simd()
     return by_lds < 1 ? 1 : (by_lds < by_regs ? by_lds : by_regs);
 }
 
-template <int KIND, int S, bool F64, int ZU, int TR, int VB>
+template <int KIND, int S, bool F64, int ZU, int TR, int VB, bool WIN = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lds_waves_per_simd<ZU, TR, VB>(), 8)))
 void bp3d_lds_kernel(BpLaunch a)
 {
@@ -500,7 +525,7 @@ void bp3d_lds_kernel(BpLaunch a)
     const uint32_t nbx = (a.nx + 15u) / 16u, nby = (a.ny + 15u) / 16u;
     const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t bx = t % nbx, rest = t / nbx;
-    bp_tile3d_lds<KIND, S, F64, ZU, TR, VB>(a, bx, rest % nby, rest / nby, threadIdx.x, tile, boxes);
+    bp_tile3d_lds<KIND, S, F64, ZU, TR, VB, WIN>(a, bx, rest % nby, rest / nby, threadIdx.x, tile, boxes);
 }
 
 // 2-D images: each block of views has its (padded) detector rows staged whole
@@ -632,6 +657,13 @@ void launch_bp3d_lds(const BpLaunch &a, hipStream_t s)
     const uint32_t zu = (a.z_unroll == 32u) ? 32u : 16u;
     const uint32_t blocks = nbx * nby * ((a.nz + zu - 1u) / zu);
     const dim3 g(blocks), b(256);
+    if constexpr (KIND == GCT_FLAT || KIND == GCT_CYLINDRICAL) {
+        if (a.zwin) {   // spiral: per-slice view windows, default variant
+            hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 2, true>),
+                               dim3(nbx * nby * ((a.nz + 15u) / 16u)), b, 0, s, a);
+            return;
+        }
+    }
     switch (zu * 8u + lds_view_block(zu, a.view_block)) {
     case 32u * 8u + 1u: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 32, 64, 1>), g, b, 0, s, a); break;
     case 32u * 8u + 2u: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 32, 64, 2>), g, b, 0, s, a); break;

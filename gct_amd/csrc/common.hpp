@@ -74,6 +74,8 @@ struct RotationInfo {
 RotationInfo rotation_info(const gct_ct_geom &g);
 void validate_geom(const gct_ct_geom &g);
 void validate_image(const gct_ct_geom &g, const gct_image_geom &img);
+// Per-slice view windows [first, last) of the spiral FDK (empty: slice skipped).
+std::vector<uint32_t> spiral_windows(const gct_ct_geom &g, const gct_image_geom &img);
 std::vector<double> ramp_taps_f64(int kind, uint32_t n, float width, float cutoff, int window,
                                   std::vector<float> *taps32);
 uint32_t next_pow2(uint32_t n);
@@ -140,6 +142,7 @@ struct BpLaunch {
     uint32_t z_unroll;
     int variant;                  // GCT_BP_DIRECT or GCT_BP_LDS
     uint32_t view_block;          // LDS kernel: views staged per block (0: default for z_unroll)
+    const uint32_t *zwin;         // spiral: [first, last) view window per z slice (device); null otherwise
 };
 
 // LDS kernel view-block depths per z-unroll: {16: 2, 3, 4 (default 2)}, {32: 1, 2 (default 2)}.

@@ -349,3 +349,63 @@ extern "C" gct_status gct_projection_matrix_decompose(const double *P, float pix
         if (focal_uv) { focal_uv[0] = fu; focal_uv[1] = fv; }
     });
 }
+
+namespace gct {
+
+// SpiralCTGeom (spiral_ct_geom.h:30-95) and the slice loop of the spiral FDK
+// (fdk_spiral.cuh:53-70), in FP32 as the reference: slice z sits at
+// leftEndPointPos(mz, nz, dz) + z dz; its centre view is the first one with
+// the smallest |source z - slice z|; the window [centre - nppr/2, + nppr) in
+// unsigned arithmetic is used only when first > 0 and last < V, otherwise the
+// slice is left empty (zero).
+std::vector<uint32_t> spiral_windows(const gct_ct_geom &g, const gct_image_geom &img)
+{
+    if (img.dim != 3) fail(GCT_ERR_INVALID_ARGUMENT, "spiral reconstruction needs a 3-D image");
+    if (is_1d(g.kind)) fail(GCT_ERR_INVALID_ARGUMENT, "spiral reconstruction needs a 2-D detector");
+    const RotationInfo r = rotation_info(g);
+    const uint32_t V = g.num_proj;
+    if (r.nppr == 0)
+        fail(GCT_ERR_RUNTIME, "Invalid spiral CT geometry: number of projections per rotation should not be zero.");
+    bool asc = true, desc = true;
+    for (uint32_t v = 1; v < V; ++v) {
+        if (g.proj_geom[v].src_pos[2] < g.proj_geom[v - 1].src_pos[2]) asc = false;
+        if (g.proj_geom[v].src_pos[2] > g.proj_geom[v - 1].src_pos[2]) desc = false;
+    }
+    if (!asc && !desc)
+        fail(GCT_ERR_RUNTIME, "Invalid spiral CT geometry: the z-coordinate of source position should be sorted "
+                              "either in ascending  or descending order.");
+    if (r.nppr >= V) fail(GCT_ERR_RUNTIME, "Invalid spiral CT geometry: fewer views than one rotation.");
+    const float pitch = std::fabs(g.proj_geom[0].src_pos[2] - g.proj_geom[r.nppr].src_pos[2]) /
+                        (2.0f * std::numbers::pi_v<float>);
+    if (!(pitch > 0.0f)) fail(GCT_ERR_RUNTIME, "Invalid spiral CT geometry: spiral pitch cannot be zero or negative.");
+    const uint32_t nz = img.z.num_samples;
+    const float dz = img.z.sampling_width;
+    const float first_pos = img.mid_point[2] - ((static_cast<float>(nz) - 1.0f) * dz / 2.0f);
+    std::vector<uint32_t> win(2u * nz, 0u);
+    for (uint32_t z = 0; z < nz; ++z) {
+        const float sp = first_pos + static_cast<float>(z) * dz;
+        uint32_t mid = 0;
+        float best = std::fabs(g.proj_geom[0].src_pos[2] - sp);
+        for (uint32_t v = 1; v < V; ++v) {
+            const float d = std::fabs(g.proj_geom[v].src_pos[2] - sp);
+            if (d < best) { best = d; mid = v; }
+        }
+        const uint32_t first = mid - r.nppr / 2u;
+        const uint32_t last = first + r.nppr;
+        if (first > 0u && last < V) { win[2u * z] = first; win[2u * z + 1u] = last; }
+    }
+    return win;
+}
+
+}  // namespace gct
+
+extern "C" gct_status gct_spiral_windows(const gct_ct_geom *geom, const gct_image_geom *img, uint32_t *win)
+{
+    return guarded([&] {
+        if (!geom || !img || !win) fail(GCT_ERR_INVALID_ARGUMENT, "null argument");
+        validate_geom(*geom);
+        validate_image(*geom, *img);
+        const std::vector<uint32_t> w = spiral_windows(*geom, *img);
+        std::copy(w.begin(), w.end(), win);
+    });
+}

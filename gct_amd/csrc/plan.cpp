@@ -204,7 +204,8 @@ void upload(gct_plan &p, DevBuf &b, const void *src, size_t bytes)
 }  // namespace
 
 std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geom &img, const gct_recon_opts &o,
-                                    uint32_t filt_views, hipStream_t stream, bool ordered)
+                                    uint32_t filt_views, hipStream_t stream, bool ordered,
+                                    const std::vector<uint32_t> *spiral_win)
 {
     validate_geom(geom);
     validate_image(geom, img);
@@ -281,7 +282,9 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
         int mode = 0;
         float value = 1.0f;
         std::vector<float> fan(p->nu, 0.0f), beta(p->V, 0.0f);
-        if (is_parallel_family(geom.kind)) {
+        if (spiral_win) {
+            // the spiral FDK applies no redundancy weighting (fdk_spiral.cuh:71-82)
+        } else if (is_parallel_family(geom.kind)) {
             const float pi_f = std::numbers::pi_v<float>;
             value = (p->rot.incr > 0.0f && p->rot.range <= (p->rot.incr + pi_f)) ? 1.0f : 0.5f;
         } else if (p->rot.range > std::numbers::pi_v<float>) {
@@ -307,6 +310,12 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
         hip_check(hipStreamSynchronize(p->stream), "plan set-up");
     }
 
+    if (spiral_win) {
+        if (!p->circular) fail(GCT_ERR_UNSUPPORTED, "spiral reconstruction needs views whose u and w do not depend on z");
+        p->spiral = true;
+        upload(*p, p->d_zwin, spiral_win->data(), spiral_win->size() * sizeof(uint32_t));
+        hip_check(hipStreamSynchronize(p->stream), "spiral windows");
+    }
     p->variant = variant;
     p->z_unroll = o.z_unroll ? o.z_unroll : (variant == GCT_BP_LDS ? 16u : 8u);
     p->view_block = o.view_block;
@@ -376,6 +385,7 @@ BpLaunch bp_args(gct_plan &p, float *vol, uint32_t first, uint32_t n, bool accum
     a.vol = vol;
     a.inv_dgamma = p.inv_dgamma;
     a.scale = p.scale;
+    a.zwin = p.spiral ? p.d_zwin.as<uint32_t>() : nullptr;
     a.accumulate = accumulate;
     a.finish = finish;
     a.z_unroll = p.z_unroll;
@@ -690,6 +700,40 @@ extern "C" gct_status gct_reconstruct(const gct_ct_geom *geom, const float *proj
 }
 
 // ---- autotuner ----------------------------------------------------------------
+
+// reconFDK for SpiralCTGeom (fdk_spiral.cuh:28-94): every view is weighted
+// and filtered once (the reference refilters each view for every slice), then
+// one backprojection launch covers all slices with their own view windows.
+extern "C" gct_status gct_reconstruct_spiral(const gct_ct_geom *geom, const float *proj, const gct_image_geom *img,
+                                             float *vol, const gct_recon_opts *opts)
+{
+    return guarded([&] {
+        if (!geom || !proj || !img || !vol) fail(GCT_ERR_INVALID_ARGUMENT, "null argument");
+        if (gct_device_count() == 0) fail(GCT_ERR_NO_DEVICE, "no HIP device");
+        gct_recon_opts o = opts_or_default(opts);
+        validate_geom(*geom);
+        validate_image(*geom, *img);
+        if (geom->kind != GCT_CYLINDRICAL && geom->kind != GCT_FLAT)
+            fail(GCT_ERR_UNSUPPORTED, "spiral reconstruction is available for cylindrical and flat detectors");
+        const std::vector<uint32_t> win = spiral_windows(*geom, *img);
+        o.bp_kernel = GCT_BP_LDS;
+        o.z_unroll = 0;
+        o.view_block = 0;
+        DeviceGuard dg(o.device);
+        auto plan = make_plan(*geom, *img, o, geom->num_proj, nullptr, false, &win);
+        hipStream_t s = nullptr;
+        hip_check(hipStreamCreate(&s), "hipStreamCreate");
+        struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamDestroy(s); } } sg{s};
+        const size_t view_bytes = size_t(geom->det_col.num_samples) * geom->det_row.num_samples * sizeof(float);
+        DevBuf dvol(volume_bytes(*img));
+        DevBuf dproj(size_t(geom->num_proj) * view_bytes);
+        hip_check(hipMemcpyAsync(dproj.p, proj, dproj.bytes, hipMemcpyHostToDevice, s), "copy projections");
+        launch_filter(filter_args(*plan, dproj.as<float>(), 0, 0, plan->V, 0), s);
+        launch_backproject(bp_args(*plan, dvol.as<float>(), 0, plan->V, false, true, 0), s);
+        hip_check(hipMemcpyAsync(vol, dvol.p, dvol.bytes, hipMemcpyDeviceToHost, s), "copy volume");
+        hip_check(hipStreamSynchronize(s), "spiral reconstruction");
+    });
+}
 
 extern "C" gct_status gct_plan_autotune(gct_plan *p, float *vol, uint32_t *best_zu, float *best_ms)
 {

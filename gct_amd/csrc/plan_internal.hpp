@@ -92,13 +92,16 @@ struct gct_plan {
     bool ordered = false;
     DevBuf d_brec, d_brec64, d_frec, d_red, d_H, d_tw, d_filt, d_views;
     DevBuf d_vol_t;   // forward projection: x/y-transposed volume, allocated on first use
+    bool spiral = false;   // spiral FDK: no redundancy weighting, per-slice view windows
+    DevBuf d_zwin;         // spiral: [first, last) per z slice
 };
 
 
 namespace gct {
 
 std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geom &img, const gct_recon_opts &o,
-                                    uint32_t filt_views, hipStream_t stream = nullptr, bool ordered = false);
+                                    uint32_t filt_views, hipStream_t stream = nullptr, bool ordered = false,
+                                    const std::vector<uint32_t> *spiral_win = nullptr);
 FilterLaunch filter_args(gct_plan &p, const float *proj, uint32_t proj_view_offset, uint32_t first, uint32_t n,
                          uint32_t filt_view_offset);
 BpLaunch bp_args(gct_plan &p, float *vol, uint32_t first, uint32_t n, bool accumulate, bool finish,

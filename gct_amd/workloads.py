@@ -165,6 +165,25 @@ def serving_mix(num_requests: int = 32, seed: int = 1005) -> list[Workload]:
     return [serving_request(rng, i) for i in range(num_requests)]
 
 
+def spiral_case(kind: int = CYLINDRICAL, n: int = 48, rotations: float = 3.0, views_per_rot: int = 64,
+                nu: int = 96, nv: int = 16, pitch_mm_per_rot: float = 6.0, seed: int = 77) -> Workload:
+    """Small helical scan: source z = -pitch * angle / 2 pi (hel_pitch in mm per
+    radian), SOD 200 mm, SDD 400 mm, 0.5 mm voxels; the volume spans the
+    middle of the helix so the central slices get a full rotation of views."""
+    views = int(rotations * views_per_rot)
+    incr = 2.0 * math.pi / views_per_rot
+    hel = pitch_mm_per_rot / (2.0 * math.pi)
+    sid, sdd = 200.0, 400.0
+    du = (0.5 / sid) if kind == CYLINDRICAL else 1.0
+    scan = circular_scan(kind, nu, du, views, incr, nv=nv, dv=1.0, det_row_center=(nu - 1) / 2.0,
+                         det_col_center=(nv - 1) / 2.0, src_det_dist=sdd, src_iso_dist=sid, hel_pitch=hel)
+    zc = -hel * incr * (views - 1) / 2.0   # source z at the middle view
+    img = Image(n, n, 0.5, 0.5, 16, 0.5, mid=(0.0, 0.0, zc))
+    ph = phantom_random(seed, 8, 3, 0.25 * n, (0.05, 0.35), (-0.02, 0.1))
+    ph[:, 2] += zc
+    return Workload(f"spiral_{kind}", scan, img, ph, 0.0, seed)
+
+
 def small_case(kind: int, n: int = 64, views: int = 120, nu: int = 96, nv: int = 80, short_scan: bool = False,
                direction: int = 1, seed: int = 42) -> Workload:
     """Small instance of every geometry for parity tests against the oracle."""

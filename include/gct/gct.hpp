@@ -232,6 +232,37 @@ struct AxialCTGeom : public CTGeom<DetRow, DetCol...> {
     using CTGeom<DetRow, DetCol...>::CTGeom;
 };
 
+// SpiralCTGeom (spiral_ct_geom.h:10-95): a 2-D detector on a helix; the
+// constructor throws std::runtime_error for zero projections per rotation,
+// unsorted source z or a non-positive pitch.
+template <typename DetRow, typename DetCol>
+struct SpiralCTGeom : public CTGeom<DetRow, DetCol> {
+    SpiralCTGeom(std::tuple<DetRow, DetCol> const &t_dg, std::vector<ProjGeom<DetRow, DetCol>> const &t_fpg)
+        : CTGeom<DetRow, DetCol>{t_dg, t_fpg}
+    {
+        // validity is the window computation's: it throws what the reference's constructor throws
+        gct_ct_geom g = this->c();
+        gct_image_geom probe{};
+        probe.dim = 3u;
+        probe.x = probe.y = probe.z = gct_sampler{1u, 1.0f};
+        uint32_t w[2];
+        const gct_status st = gct_spiral_windows(&g, &probe, w);
+        if (st != GCT_OK) throw std::runtime_error{gct_last_error()};
+    }
+    // [first, last) views of the rotation centred on the slice at slice_pos_mm.
+    std::pair<unsigned, unsigned> firstAndLastProjIndices(float slice_pos_mm) const noexcept
+    {
+        unsigned mid = 0;
+        float best = std::fabs(this->fpg[0].src_pos.z.val - slice_pos_mm);
+        for (unsigned v = 1; v < this->fpg.size(); ++v) {
+            const float d = std::fabs(this->fpg[v].src_pos.z.val - slice_pos_mm);
+            if (d < best) { best = d; mid = v; }
+        }
+        const unsigned first = mid - this->rg.num_proj_per_rot / 2u;
+        return {first, first + this->rg.num_proj_per_rot};
+    }
+};
+
 // ReconImgGeom<2|3> (src/cpp/geom/recon_img_geom.h:12-30).
 template <size_t Dim> struct ReconImgGeom;
 template <> struct ReconImgGeom<2u> {
@@ -458,6 +489,29 @@ inline void reconstruct(Geom::AxialCTGeom<Geom::DetFanLine, Geom::DetFanLine> co
                         Data::ProjectionStack<float, 2u> const &proj, Data::ReconImg<float, 3u> &vol,
                         gct_recon_opts const *opts = nullptr)
 { Detail::reconstructImpl(cg, proj, vol, opts); }
+
+// Spiral FDK (recon_wrappers.cu:58-63, fdk_spiral.cuh:28-94); the flat-panel
+// overload is an extension.
+inline void reconstruct(Geom::SpiralCTGeom<Geom::DetFanArc, Geom::DetFanLine> const &cg,
+                        Data::ProjectionStack<float, 2u> const &proj, Data::ReconImg<float, 3u> &vol,
+                        gct_recon_opts const *opts = nullptr)
+{
+    gct_ct_geom g = cg.c();
+    gct_image_geom im = vol.c();
+    std::vector<float> out(vol.numElements());
+    GCT::Detail::check(gct_reconstruct_spiral(&g, proj.elements().data(), &im, out.data(), opts));
+    vol.setData(std::move(out));
+}
+inline void reconstruct(Geom::SpiralCTGeom<Geom::DetFanLine, Geom::DetFanLine> const &cg,
+                        Data::ProjectionStack<float, 2u> const &proj, Data::ReconImg<float, 3u> &vol,
+                        gct_recon_opts const *opts = nullptr)
+{
+    gct_ct_geom g = cg.c();
+    gct_image_geom im = vol.c();
+    std::vector<float> out(vol.numElements());
+    GCT::Detail::check(gct_reconstruct_spiral(&g, proj.elements().data(), &im, out.data(), opts));
+    vol.setData(std::move(out));
+}
 
 }  // namespace Recon
 // ---- array files (src/cpp/io): readers, writers, projection and image I/O -------

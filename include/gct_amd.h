@@ -349,6 +349,22 @@ gct_status gct_phantom_project_device(const gct_ct_geom *geom, int dim, const do
 gct_status gct_phantom_rasterize_device(const gct_image_geom *img, const double *params, uint32_t count,
                                         float *vol_dev, void *stream);
 
+/* ---- spiral FDK --------------------------------------------------------- */
+
+/* SpiralCTGeom::firstAndLastProjIndices for every slice of img
+ * (spiral_ct_geom.h:30-95, fdk_spiral.cuh:53-70): win[2z], win[2z+1] =
+ * [first, last) views of slice z, or 0, 0 when the reference skips the slice.
+ * GCT_ERR_RUNTIME for a geometry SpiralCTGeom rejects (zero projections per
+ * rotation, unsorted source z, non-positive pitch). */
+gct_status gct_spiral_windows(const gct_ct_geom *geom, const gct_image_geom *img, uint32_t *win);
+/* Recon::CUDA::reconstruct(SpiralCTGeom<DetFanArc, DetFanLine>, ...)
+ * (recon_wrappers.cu:58-63 -> fdk_spiral.cuh:28-94): cosine weighting and the
+ * Ram-Lak ramp (no redundancy weighting), each slice backprojected from its
+ * own window, x rot_ang_incr.  Cylindrical (the reference's case) and flat
+ * detectors; host buffers in and out, synchronous.  opts: storage, device. */
+gct_status gct_reconstruct_spiral(const gct_ct_geom *geom, const float *proj, const gct_image_geom *img,
+                                  float *vol, const gct_recon_opts *opts);
+
 /* ---- array files: MetaImage, Dennerlein, raw (the reference's src/cpp/io) -- */
 
 /* ArrayElemType (metadata.h:9-21). */

@@ -365,6 +365,44 @@ void or_filter_rows_direct_f64(unsigned nrows, unsigned n, const double *taps, c
 
 /* ---- precision-generic stages -------------------------------------------- */
 
+/* ---- spiral view windows (spiral_ct_geom.h:30-95, fdk_spiral.cuh:53-70) ----
+ * SpiralCTGeom's checks: projections per rotation != 0, source z sorted
+ * (non-decreasing or non-increasing), pitch |z(0) - z(nppr)| / 2 pi > 0.
+ * Slice s sits at leftEndPointPos(mz, nz, dz) + s dz (FP32); its centre view is
+ * the first view with the smallest |src z - slice z| (sliceProjIndex); the
+ * window [mid - nppr/2, + nppr) in unsigned arithmetic is used only when
+ * first > 0 and last < V (fdk_spiral.cuh:68), otherwise the slice stays 0. */
+int or_spiral_windows(const or_scan *s, const or_image *img, unsigned *win)
+{
+    or_rot r;
+    unsigned V = s->nviews;
+    if (img->dim != 3 || or_rotation_geom(s, &r, NULL, NULL) != 0) return -1;
+    if (r.nppr == 0 || r.nppr >= V) return -1;
+    int asc = 1, desc = 1;
+    for (unsigned v = 1; v < V; ++v) {
+        if (s->views[v].src_pos[2] < s->views[v - 1].src_pos[2]) asc = 0;
+        if (s->views[v].src_pos[2] > s->views[v - 1].src_pos[2]) desc = 0;
+    }
+    if (!asc && !desc) return -1;
+    float pitch = fabsf(s->views[0].src_pos[2] - s->views[r.nppr].src_pos[2]) / (2.0f * OR_PI_F);
+    if (!(pitch > 0.0f)) return -1;
+    float first_pos = img->mz - (((float)img->nz - 1.0f) * img->dz / 2.0f);
+    for (unsigned z = 0; z < img->nz; ++z) {
+        float sp = first_pos + (float)z * img->dz;
+        unsigned mid = 0;
+        float best = fabsf(s->views[0].src_pos[2] - sp);
+        for (unsigned v = 1; v < V; ++v) {
+            float d = fabsf(s->views[v].src_pos[2] - sp);
+            if (d < best) { best = d; mid = v; }
+        }
+        unsigned first = mid - r.nppr / 2u;
+        unsigned last = first + r.nppr;
+        if (first > 0u && last < V) { win[2 * z] = first; win[2 * z + 1] = last; }
+        else { win[2 * z] = 0u; win[2 * z + 1] = 0u; }
+    }
+    return 0;
+}
+
 #define REAL float
 #define SFX(x) x##_f32
 #define RSQRT sqrtf

@@ -101,6 +101,17 @@ int or_fdk_f32(const or_scan *scan, const or_image *img, float cutoff, int windo
 int or_fdk_f64(const or_scan *scan, const or_image *img, float cutoff, int window,
                const float *proj, double *vol, unsigned z0, unsigned z1, int nthreads);
 
+/* Spiral FDK (fdk_spiral.cuh:28-94): per-slice view windows [win[2z], win[2z+1])
+ * (empty when the reference skips the slice); -1 for a geometry SpiralCTGeom
+ * rejects (spiral_ct_geom.h:30-53). */
+int or_spiral_windows(const or_scan *scan, const or_image *img, unsigned *win);
+/* Slices [z0, z1) of the spiral reconstruction: cosine weight, Ram-Lak ramp,
+ * backprojection of each slice's window, x dtheta. */
+int or_fdk_spiral_f32(const or_scan *scan, const or_image *img, const float *proj, float *vol,
+                      unsigned z0, unsigned z1, int nthreads);
+int or_fdk_spiral_f64(const or_scan *scan, const or_image *img, const float *proj, double *vol,
+                      unsigned z0, unsigned z1, int nthreads);
+
 /* Joseph forward projection (no reference counterpart: the reference's
  * forward_proj.cuh is dead code).  Rays follow the backprojector's conventions. */
 int or_joseph_f64(const or_scan *scan, const or_image *img, const double *vol, double *proj, int nthreads);

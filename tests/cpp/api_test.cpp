@@ -132,6 +132,36 @@ int main(int argc, char **argv)
         CHECK(threw);
     }
 
+    // spiral geometry (spiral_ct_geom.h): 3 rotations of 64 views on a helix of 6 mm per rotation
+    {
+        ConeBeamProjGeomParams p;
+        p.rot.init_ang = Radians{0.0f};
+        p.rot.ang_incr = Radians{2.0f * std::numbers::pi_v<float> / 64.0f};
+        p.rot.num_proj = 192u;
+        p.rot.hel_pitch = 6.0f / (2.0f * std::numbers::pi_v<float>);
+        p.det_row_center = 47.5_pix;
+        p.det_col_center = 7.5_pix;
+        p.src_iso_dist = 200.0_mm;
+        p.src_det_dist = 400.0_mm;
+        std::vector<ProjGeom<DetFanArc, DetFanLine>> v;
+        for (unsigned i = 0; i < 192u; ++i) v.push_back(ProjGeomFactory<DetFanArc, DetFanLine>::create(p, i));
+        const auto dets = std::make_tuple(DetFanArc{{96u, Radians{0.0025f}}}, DetFanLine{{16u, 1.0_mm}});
+        SpiralCTGeom<DetFanArc, DetFanLine> sg{dets, v};
+        CHECK(sg.rotationGeom().num_proj_per_rot == 64u);
+        const auto [f, l] = sg.firstAndLastProjIndices(v[100].src_pos.z.val);
+        CHECK(f == 68u && l == 132u);
+        // no pitch: the constructor throws (spiral pitch cannot be zero)
+        p.rot.hel_pitch = 0.0f;
+        std::vector<ProjGeom<DetFanArc, DetFanLine>> c;
+        for (unsigned i = 0; i < 192u; ++i) c.push_back(ProjGeomFactory<DetFanArc, DetFanLine>::create(p, i));
+        bool threw = false;
+        try {
+            SpiralCTGeom<DetFanArc, DetFanLine> bad{dets, c};
+        } catch (std::runtime_error const &) {
+            threw = true;
+        }
+        CHECK(threw);
+    }
     // array files through GCT::IO (meta_img_file_writer_tests.cpp, mhd_file_reader_tests.cpp):
     // the 3 x 2 x 4 iota fixture written as .mha and .mhd and read back, a series, a DEN image
     {

@@ -58,6 +58,9 @@ def oracle() -> C.CDLL:
             "or_fdk_f32": [P, P, F, I, P, P, U, U, I],
             "or_fdk_f64": [P, P, F, I, P, P, U, U, I],
             "or_joseph_f64": [P, P, P, P, I],
+            "or_spiral_windows": [P, P, P],
+            "or_fdk_spiral_f32": [P, P, P, P, U, U, I],
+            "or_fdk_spiral_f64": [P, P, P, P, U, U, I],
         }
         for name, args in sigs.items():
             f = getattr(_o, name)
@@ -186,6 +189,25 @@ def fdk(scan, img, proj, cutoff=1.0, window=0, z0=0, z1=None, f64=False, nthread
             z0, z1, nthreads)
     if rc != 0:
         raise RuntimeError(f"oracle FDK failed ({rc})")
+    return vol
+
+
+def spiral_windows(scan, img):
+    win = np.zeros((img.nz, 2), np.uint32)
+    if oracle().or_spiral_windows(_sref(scan), _iref(img), win.ctypes.data) != 0:
+        raise RuntimeError("invalid spiral geometry")
+    return win
+
+
+def fdk_spiral(scan, img, proj, z0=0, z1=None, f64=False, nthreads=8):
+    dt = np.float64 if f64 else np.float32
+    proj = np.ascontiguousarray(proj, np.float32)
+    vol = np.zeros(img.shape, dt)
+    z1 = img.nz if z1 is None else z1
+    fn = oracle().or_fdk_spiral_f64 if f64 else oracle().or_fdk_spiral_f32
+    rc = fn(_sref(scan), _iref(img), proj.ctypes.data, vol.ctypes.data, z0, z1, nthreads)
+    if rc != 0:
+        raise RuntimeError(f"oracle spiral FDK failed ({rc})")
     return vol
 
 
