@@ -17,7 +17,8 @@ from pathlib import Path
 import numpy as np
 
 _PKG = Path(__file__).resolve().parent
-LIB_PATH = _PKG / "libgct_amd.so"
+# GCT_AMD_LIB selects another build of the library (default: the in-tree one).
+LIB_PATH = Path(os.environ["GCT_AMD_LIB"]) if os.environ.get("GCT_AMD_LIB") else _PKG / "libgct_amd.so"
 
 # gct_geom_kind
 PARALLEL_1D, FAN_ARC_1D, FAN_LINE_1D, PARALLEL_2D, CONE_PARALLEL, CYLINDRICAL, FLAT = range(7)

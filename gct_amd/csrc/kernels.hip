@@ -401,6 +401,7 @@ __device__ float joseph_ray(const FpLaunch &a, const float o[3], const float d[3
     const size_t pstep = stride[ax];
     float acc = 0.0f;
     if (!is3) {
+#pragma unroll 8
         for (int i = i0; i <= i1; ++i, plane += pstep) {
             const float fa = fmaf(static_cast<float>(i), dfa, fa0);
             const float ffa = floorf(fa);
@@ -415,7 +416,7 @@ __device__ float joseph_ray(const FpLaunch &a, const float o[3], const float d[3
     const float fb0 = static_cast<float>((o[ob] + t0 * d[ob] - first[ob]) / sp[ob]);
     const float dfb = static_cast<float>(dt * d[ob] / sp[ob]);
     const uint32_t nb = static_cast<uint32_t>(n[ob]), sb = stride[ob];
-#pragma unroll 2
+#pragma unroll 8   // independent planes: eight in flight per thread
     for (int i = i0; i <= i1; ++i, plane += pstep) {
         const float fi = static_cast<float>(i);
         const float fa = fmaf(fi, dfa, fa0), fb = fmaf(fi, dfb, fb0);
