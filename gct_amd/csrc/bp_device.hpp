@@ -49,24 +49,32 @@ struct Sample {
     float alpha;        // horizontal fraction
 };
 
-// Clamping the coordinate to [-1, n] keeps every tap inside the padded
-// buffer; any coordinate outside the detector then lands on zero padding,
-// which reproduces the border-zero texture (nd_array_texture.cuh:43-50).
-__device__ inline Sample col_sample(float col, float nu)
+// Taps are addressed by padded coordinates (detector coordinate + 1).
+// Clamping one to [0, n + 1] keeps every tap inside the padded buffer; any
+// coordinate outside the detector then lands on zero padding, which
+// reproduces the border-zero texture (nd_array_texture.cuh:43-50).  The
+// clamped coordinate is non-negative, so truncation is its floor and
+// v_fract_f32 gives its fraction exactly.
+__device__ inline Sample col_sample(float col, float nup)
 {
-    const float c = __builtin_amdgcn_fmed3f(col, -1.0f, nu);
-    const float f = floorf(c);
-    return Sample{static_cast<uint32_t>(static_cast<int>(f) + 1), c - f};
+    const float c = __builtin_amdgcn_fmed3f(col + 1.0f, 0.0f, nup);
+    return Sample{static_cast<uint32_t>(c), __builtin_amdgcn_fractf(c)};
 }
 
-template <int S>
-__device__ inline float bilinear(const typename Stor<S>::T *__restrict__ p, uint32_t pitch, Sample cs, float row,
-                                 float nvf)
+// The largest float below x > 0.
+__device__ inline float below(float x)
 {
-    const float r = __builtin_amdgcn_fmed3f(row, -1.0f, nvf);
-    const float fr = floorf(r);
-    const float beta = r - fr;
-    const uint32_t off = static_cast<uint32_t>(static_cast<int>(fr) + 1) * pitch + cs.colbase;
+    return __int_as_float(__float_as_int(x) - 1);
+}
+
+// `rowp` is the padded row coordinate (row + 1), clamped like a column.
+template <int S>
+__device__ inline float bilinear(const typename Stor<S>::T *__restrict__ p, uint32_t pitch, Sample cs, float rowp,
+                                 float nvp)
+{
+    const float r = __builtin_amdgcn_fmed3f(rowp, 0.0f, nvp);
+    const float beta = __builtin_amdgcn_fractf(r);
+    const uint32_t off = static_cast<uint32_t>(r) * pitch + cs.colbase;
     const float t0 = Stor<S>::ld(p, off), t1 = Stor<S>::ld(p, off + 1u);
     const float b0 = Stor<S>::ld(p, off + pitch), b1 = Stor<S>::ld(p, off + pitch + 1u);
     const float top = fmaf(cs.alpha, t1 - t0, t0);
@@ -102,28 +110,28 @@ __device__ inline ColSetup setup_column(const BpLaunch &a, uint32_t v, float fx,
         // centre); evaluating it in FP64 once per column keeps it as accurate
         // as the reference's FP32 chain t = x cos + y sin, col = t / du + cu.
         const BpRec64 &r = a.rec64[v];
-        s.col = static_cast<float>(r.u[0] + r.u[1] * static_cast<double>(fx) + r.u[2] * static_cast<double>(fy));
+        s.col = static_cast<float>(fma(r.u[2], static_cast<double>(fy), fma(r.u[1], static_cast<double>(fx), r.u[0])));
         s.w = 1.0f;
-        s.row0 = static_cast<float>(r.v[0] + r.v[3] * static_cast<double>(fz));
+        s.row0 = static_cast<float>(fma(r.v[3], static_cast<double>(fz), r.v[0]));
         s.drow = static_cast<float>(r.v[3]);
     } else if constexpr (KIND == GCT_FLAT) {
         if constexpr (F64) {
             const BpRec64 &r = a.rec64[v];
             const double X = fx, Y = fy, Z = fz;
-            const double nw = r.w[0] + r.w[1] * X + r.w[2] * Y + r.w[3] * Z;
-            const double nu = r.u[0] + r.u[1] * X + r.u[2] * Y + r.u[3] * Z;
-            const double nv = r.v[0] + r.v[1] * X + r.v[2] * Y + r.v[3] * Z;
+            const double nw = fma(r.w[3], Z, fma(r.w[2], Y, fma(r.w[1], X, r.w[0])));
+            const double nu = fma(r.u[3], Z, fma(r.u[2], Y, fma(r.u[1], X, r.u[0])));
+            const double nv = fma(r.v[3], Z, fma(r.v[2], Y, fma(r.v[1], X, r.v[0])));
             const double rd = 1.0 / nw;
-            s.col = static_cast<float>(r.k[0] + nu * rd);
+            s.col = static_cast<float>(fma(nu, rd, r.k[0]));
             s.w = static_cast<float>(rd * rd);
-            s.row0 = static_cast<float>(r.k[1] + nv * rd);
+            s.row0 = static_cast<float>(fma(nv, rd, r.k[1]));
             s.drow = static_cast<float>(r.v[3] * rd);
         } else {
             const BpRec &r = a.rec[v];
             const float nw = fmaf(r.w[3], fz, fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0])));
             const float nu = fmaf(r.u[3], fz, fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0])));
             const float nv = fmaf(r.v[3], fz, fmaf(r.v[2], fy, fmaf(r.v[1], fx, r.v[0])));
-            const float rd = 1.0f / nw;
+            const float rd = __builtin_amdgcn_rcpf(nw);   // v_rcp_f32 (1 ulp); nw = 0 gives inf as 1 / nw does
             s.col = fmaf(nu, rd, r.k[0]);
             s.w = rd * rd;
             s.row0 = fmaf(nv, rd, r.k[1]);
@@ -133,13 +141,13 @@ __device__ inline ColSetup setup_column(const BpLaunch &a, uint32_t v, float fx,
         if constexpr (F64) {
             const BpRec64 &r = a.rec64[v];
             const double X = fx, Y = fy, Z = fz;
-            const double nw = r.w[0] + r.w[1] * X + r.w[2] * Y + r.w[3] * Z;
-            const double nu = r.u[0] + r.u[1] * X + r.u[2] * Y + r.u[3] * Z;
-            const double nv = r.v[0] + r.v[1] * X + r.v[2] * Y + r.v[3] * Z;
-            const double rho2 = nu * nu + nw * nw;
+            const double nw = fma(r.w[3], Z, fma(r.w[2], Y, fma(r.w[1], X, r.w[0])));
+            const double nu = fma(r.u[3], Z, fma(r.u[2], Y, fma(r.u[1], X, r.u[0])));
+            const double nv = fma(r.v[3], Z, fma(r.v[2], Y, fma(r.v[1], X, r.v[0])));
+            const double rho2 = fma(nu, nu, nw * nw);
             const double rr = 1.0 / sqrt(rho2);
-            s.col = static_cast<float>(r.k[0] + atan2(nu, nw) * static_cast<double>(a.inv_dgamma));
-            s.row0 = static_cast<float>(r.k[1] + r.k[2] * nv * rr);
+            s.col = static_cast<float>(fma(atan2(nu, nw), static_cast<double>(a.inv_dgamma), r.k[0]));
+            s.row0 = static_cast<float>(fma(r.k[2] * nv, rr, r.k[1]));
             s.drow = static_cast<float>(r.k[2] * r.v[3] * rr);
             s.rho2 = static_cast<float>(rho2);
             s.nv0 = static_cast<float>(nv);
@@ -193,7 +201,8 @@ __device__ inline void bp_tile3d(const BpLaunch &a, uint32_t bx, uint32_t by, ui
     for (int k = 0; k < ZU; ++k) acc[k] = (a.accumulate && static_cast<uint32_t>(k) < zn) ? vp[k * slice] : 0.0f;
 
     const float fx = static_cast<float>(ix), fy = static_cast<float>(iy), fz = static_cast<float>(iz0);
-    const float nuf = static_cast<float>(a.nu), nvf = static_cast<float>(a.nv);
+    const float nuf = static_cast<float>(a.nu), nup = static_cast<float>(a.nu + 1u);
+    const float nvp = static_cast<float>(a.nv + 1u);
     const size_t view_elems = static_cast<size_t>(a.rows_pv) * a.pitch;
     const T *base = static_cast<const T *>(a.filt);
     const uint32_t vend = a.first_view + a.num_views;
@@ -203,12 +212,12 @@ __device__ inline void bp_tile3d(const BpLaunch &a, uint32_t bx, uint32_t by, ui
             // set up at z index 0 and step by the absolute index, so the
             // result does not depend on how z is split into tiles
             const ColSetup s = setup_column<KIND, F64>(a, v, fx, fy, 0.0f, nuf);
-            const Sample cs = col_sample(s.col, nuf);
+            const Sample cs = col_sample(s.col, nup);
+            const float rowp0 = s.row0 + 1.0f;
 #pragma unroll
             for (int k = 0; k < ZU; ++k) {
                 const float zk = fz + static_cast<float>(k);
-                const float row = fmaf(zk, s.drow, s.row0);
-                const float val = bilinear<S>(p, a.pitch, cs, row, nvf);
+                const float val = bilinear<S>(p, a.pitch, cs, fmaf(zk, s.drow, rowp0), nvp);
                 float w = s.w;
                 if constexpr (KIND == GCT_CYLINDRICAL) {
                     const float nvk = fmaf(zk, s.dnv, s.nv0);
@@ -220,7 +229,7 @@ __device__ inline void bp_tile3d(const BpLaunch &a, uint32_t bx, uint32_t by, ui
 #pragma unroll
             for (int k = 0; k < ZU; ++k) {
                 const ColSetup s = setup_column<KIND, F64>(a, v, fx, fy, fz + static_cast<float>(k), nuf);
-                const float val = bilinear<S>(p, a.pitch, col_sample(s.col, nuf), s.row0, nvf);
+                const float val = bilinear<S>(p, a.pitch, col_sample(s.col, nup), s.row0 + 1.0f, nvp);
                 float w = s.w;
                 if constexpr (KIND == GCT_CYLINDRICAL) w = s.w * __builtin_amdgcn_rcpf(fmaf(s.nv0, s.nv0, s.rho2));
                 acc[k] = fmaf(val, w, acc[k]);
@@ -242,7 +251,7 @@ __device__ inline void bp_tile2d(const BpLaunch &a, uint32_t ix, uint32_t iy)
     float *vp = a.vol + static_cast<size_t>(iy) * a.nx + ix;
     float acc = a.accumulate ? *vp : 0.0f;
     const float fx = static_cast<float>(ix), fy = static_cast<float>(iy);
-    const float nuf = static_cast<float>(a.nu);
+    const float nup = static_cast<float>(a.nu + 1u);
     const size_t view_elems = static_cast<size_t>(a.rows_pv) * a.pitch;
     const T *base = static_cast<const T *>(a.filt);
     const uint32_t vend = a.first_view + a.num_views;
@@ -265,7 +274,7 @@ __device__ inline void bp_tile2d(const BpLaunch &a, uint32_t ix, uint32_t iy)
             col = fmaf(atan2f(nu, nw), a.inv_dgamma, r.k[0]);
             w = r.k[3] / fmaf(nu, nu, nw * nw);
         }
-        const float val = linear<S>(p, a.pitch, col_sample(col, nuf));
+        const float val = linear<S>(p, a.pitch, col_sample(col, nup));
         acc = fmaf(val, w, acc);
     }
     *vp = a.finish ? acc * a.scale : acc;
@@ -316,6 +325,13 @@ __global__ __launch_bounds__(64) void bp2d_kernel(BpLaunch a)
 
 constexpr int kTP = 48;   // staged (value, difference) pairs per row
 
+// Lane n + 1's value (a wavefront shift by one lane, one DPP move; lane 63
+// reads zero).
+__device__ inline float next_lane(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xF, 0xF, true));
+}
+
 struct Box {
     int clo, rlo, ncols, nrows;   // ncols == 0: the view is read from HBM
 };
@@ -354,6 +370,80 @@ __device__ inline void tile_boxes(const BpLaunch &a, uint32_t bx, uint32_t by, u
     boxes[j] = b;
 }
 
+// Staging one view's box into LDS as (value, right neighbour - value) pairs.
+// The wave's rows are rw, rw + RSTEP, ...
+// (wave-uniform; with HALF, lanes 32-63 take the row below), so row guards are
+// scalar branches, load addresses are a uniform row pointer plus the lane's
+// offset, and the LDS stores take their row step as an immediate offset.  A
+// row of m pairs needs m + 1 values: each lane loads one and takes its right
+// neighbour from the next lane.
+template <int S, int RSTEP, int NIT, bool HALF>
+__device__ inline void stage_load(const typename Stor<S>::T *p, uint32_t pitch, int rw, int m, int nrows,
+                                  uint32_t lane, float *v)
+{
+    using T = typename Stor<S>::T;
+    const int c = HALF ? static_cast<int>(lane & 31u) : static_cast<int>(lane);
+    const int h = HALF ? static_cast<int>(lane >> 5) : 0;
+    const uint32_t loff = static_cast<uint32_t>(h) * pitch + static_cast<uint32_t>(c);
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int r = rw + it * RSTEP;
+        v[it] = 0.0f;
+        if (r < nrows) {
+            const T *pr = p + static_cast<size_t>(r) * pitch;
+            if (c <= m && r + h < nrows) v[it] = Stor<S>::ld(pr, loff);
+        }
+    }
+}
+
+template <int RSTEP, int NIT, bool HALF>
+__device__ inline void stage_store(float2 *tl, int rw, int m, int nrows, uint32_t lane, const float *v)
+{
+    const int c = HALF ? static_cast<int>(lane & 31u) : static_cast<int>(lane);
+    const int h = HALF ? static_cast<int>(lane >> 5) : 0;
+    float2 *tw = tl + (rw + h) * kTP + c;
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int r = rw + it * RSTEP;
+        if (r < nrows) {
+            const float d = next_lane(v[it]) - v[it];
+            if (c < m && r + h < nrows) tw[it * RSTEP * kTP] = make_float2(v[it], d);
+        }
+    }
+}
+
+// Stages view block [vb0, vb0 + nvb) with boxes `bx`, one view at a time
+// (all loads of a view are issued before its first store).  Box row r,
+// column c is padded pixel (rlo + 1 + r, clo + 1 + c); a box of fewer than
+// 32 pairs per row is staged two rows per wave.
+template <int S, int TR, int VB>
+__device__ inline void stage_block(const typename Stor<S>::T *base, size_t view_elems, uint32_t pitch, uint32_t vb0,
+                                   uint32_t nvb, const Box *bx, float2 *tile, int wave, uint32_t lane)
+{
+    for (uint32_t j = 0; j < nvb; ++j) {
+        const int ncols = __builtin_amdgcn_readfirstlane(bx[j].ncols);
+        if (ncols == 0) continue;
+        const int clo = __builtin_amdgcn_readfirstlane(bx[j].clo);
+        const int rlo = __builtin_amdgcn_readfirstlane(bx[j].rlo);
+        const int nrows = __builtin_amdgcn_readfirstlane(bx[j].nrows);
+        const typename Stor<S>::T *p = base + static_cast<size_t>(vb0 + j) * view_elems +
+                                       (static_cast<size_t>(rlo + 1) * pitch + static_cast<uint32_t>(clo + 1));
+        float2 *tl = tile + j * (TR * kTP);
+        const int m = ncols - 1;
+        if (m < 32) {
+            constexpr int kN = (TR + 7) / 8;
+            float v[kN];
+            stage_load<S, 8, kN, true>(p, pitch, 2 * wave, m, nrows, lane, v);
+            stage_store<8, kN, true>(tl, 2 * wave, m, nrows, lane, v);
+        } else {
+            constexpr int kW = (TR + 3) / 4;
+            float v[kW];
+            stage_load<S, 4, kW, false>(p, pitch, wave, m, nrows, lane, v);
+            stage_store<4, kW, false>(tl, wave, m, nrows, lane, v);
+        }
+    }
+}
+
 // One 16 x 16 x ZU tile (bx, by, bz) of a circular 3-D scan; every thread of
 // the 256-thread workgroup calls it (it holds workgroup barriers).  `tile`
 // and `boxes` are the workgroup's LDS staging area of VB * TR * kTP pairs and
@@ -378,7 +468,8 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
         acc[k] = (active && a.accumulate && static_cast<uint32_t>(k) < zn) ? vp[k * slice] : 0.0f;
 
     const float fx = static_cast<float>(ix), fy = static_cast<float>(iy), fz = static_cast<float>(iz0);
-    const float nuf = static_cast<float>(a.nu), nvf = static_cast<float>(a.nv);
+    const float nuf = static_cast<float>(a.nu), nup = static_cast<float>(a.nu + 1u);
+    const float nvp = static_cast<float>(a.nv + 1u);
     const size_t view_elems = static_cast<size_t>(a.rows_pv) * a.pitch;
     const T *base = static_cast<const T *>(a.filt);
     uint32_t vfirst = a.first_view, vend = a.first_view + a.num_views;
@@ -399,41 +490,13 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
         vend = min(vend, hi);
     }
 
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(tid >> 6));
+    const uint32_t lane = tid & 63u;
     for (uint32_t vb0 = vfirst; vb0 < vend; vb0 += VB) {
         const uint32_t nvb = min(static_cast<uint32_t>(VB), vend - vb0);
         tile_boxes<KIND, ZU>(a, bx, by, bz, vb0, nvb, TR, boxes, tid);
         __syncthreads();
-        for (uint32_t j = 0; j < nvb; ++j) {
-            const Box b = boxes[j];
-            if (b.ncols == 0) continue;
-            // Box row r, column c is padded pixel (rlo + 1 + r, clo + 1 + c): one
-            // uniform base per view, 32-bit offsets.  A row of m pairs needs m + 1
-            // values; each lane loads one and takes its right neighbour from the
-            // next lane, and every load of the view is issued before the first use.
-            const T *p = base + static_cast<size_t>(vb0 + j) * view_elems +
-                         (static_cast<size_t>(b.rlo + 1) * a.pitch + static_cast<uint32_t>(b.clo + 1));
-            float2 *tl = tile + j * (TR * kTP);
-            const int m = b.ncols - 1;            // staged pairs per row
-            const bool narrow = m < 32;           // two rows per wave, 32 lanes each
-            const int c = narrow ? static_cast<int>(tid & 31u) : static_cast<int>(tid & 63u);
-            const int r0 = narrow ? static_cast<int>(tid >> 5) : static_cast<int>(tid >> 6);
-            const int rstep = narrow ? 8 : 4;
-            constexpr int kIt = (TR + 3) / 4;     // row steps of the wide mode (the narrow mode uses half)
-            float v[kIt];
-#pragma unroll
-            for (int it = 0; it < kIt; ++it) {
-                const int r = r0 + it * rstep;
-                v[it] = (r < b.nrows && c <= m)
-                            ? Stor<S>::ld(p, static_cast<uint32_t>(r) * a.pitch + static_cast<uint32_t>(c))
-                            : 0.0f;
-            }
-#pragma unroll
-            for (int it = 0; it < kIt; ++it) {
-                const int r = r0 + it * rstep;
-                const float right = __shfl_down(v[it], 1);
-                if (r < b.nrows && c < m) tl[r * kTP + c] = make_float2(v[it], right - v[it]);
-            }
-        }
+        stage_block<S, TR, VB>(base, view_elems, a.pitch, vb0, nvb, boxes, tile, wave, lane);
         __syncthreads();
         if (active) {
             for (uint32_t j = 0; j < nvb; ++j) {
@@ -441,27 +504,31 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                 const ColSetup s = setup_column<KIND, F64>(a, v, fx, fy, 0.0f, nuf);
                 const Box b = boxes[j];
                 if (b.ncols != 0) {
-                    const float cl = __builtin_amdgcn_fmed3f(s.col, static_cast<float>(b.clo),
-                                                             static_cast<float>(b.clo + b.ncols - 2));
-                    const float cf = floorf(cl);
-                    const float alpha = cl - cf;
-                    // Pair (r, c) of the box sits at byte 8 (j TR kTP + (r - rlo) kTP + (c - clo)).
-                    // The per-view terms fold into one base; the row term is added in float,
-                    // which is exact because every offset is an integer below 2^24.
-                    // (signed: the base is negative when the box starts below row 0 of view j's slot)
-                    const int ebase = static_cast<int>(j) * (TR * kTP) + (static_cast<int>(cf) - b.clo) - b.rlo * kTP;
-                    const float obase = static_cast<float>(8 * ebase);
-                    const float rlo = static_cast<float>(b.rlo), rhi = static_cast<float>(b.rlo + b.nrows - 2);
+                    // Padded coordinates, as in col_sample / bilinear.  The box holds padded
+                    // columns clo + 1 .. clo + ncols and rows rlo + 1 .. rlo + nrows; clamping
+                    // below the last of each keeps both taps inside it, and never binds for a
+                    // coordinate of this tile, which the box covers with a margin.
+                    const float clp = __builtin_amdgcn_fmed3f(s.col + 1.0f, static_cast<float>(b.clo + 1),
+                                                              below(static_cast<float>(b.clo + b.ncols)));
+                    const float alpha = __builtin_amdgcn_fractf(clp);
+                    // Pair (r, c) of the box sits at byte 8 (j TR kTP + r kTP + c).  The per-view
+                    // terms fold into one base (modulo 2^32: it is negative when the box starts
+                    // below row 0 of view j's slot); the padded row index is added with one
+                    // 24-bit multiply-add.
+                    const uint32_t obase = 8u * static_cast<uint32_t>(static_cast<int>(j) * (TR * kTP) +
+                                                                      (static_cast<int>(clp) - (b.clo + 1)) -
+                                                                      (b.rlo + 1) * kTP);
+                    const float rlo = static_cast<float>(b.rlo + 1), rhi = below(static_cast<float>(b.rlo + b.nrows));
+                    const float rowp0 = s.row0 + 1.0f;
 #pragma unroll
                     for (int k = 0; k < ZU; ++k) {
                         if constexpr (WIN) {
                             if (v < wl[k] || v >= wh[k]) continue;
                         }
                         const float zk = fz + static_cast<float>(k);
-                        const float row = __builtin_amdgcn_fmed3f(fmaf(zk, s.drow, s.row0), rlo, rhi);
-                        const float rf = floorf(row);
-                        const float beta = row - rf;
-                        const uint32_t off = static_cast<uint32_t>(fmaf(rf, static_cast<float>(8 * kTP), obase));
+                        const float row = __builtin_amdgcn_fmed3f(fmaf(zk, s.drow, rowp0), rlo, rhi);
+                        const float beta = __builtin_amdgcn_fractf(row);
+                        const uint32_t off = __umul24(static_cast<uint32_t>(row), 8u * kTP) + obase;
                         const float2 *q = reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(tile) + off);
                         const float2 tp = q[0], bt = q[kTP];
                         const float top = fmaf(alpha, tp.y, tp.x);
@@ -476,15 +543,15 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                     }
                 } else {
                     const T *p = base + static_cast<size_t>(v) * view_elems;
-                    const Sample cs = col_sample(s.col, nuf);
+                    const Sample cs = col_sample(s.col, nup);
+                    const float rowp0 = s.row0 + 1.0f;
 #pragma unroll
                     for (int k = 0; k < ZU; ++k) {
                         if constexpr (WIN) {
                             if (v < wl[k] || v >= wh[k]) continue;
                         }
                         const float zk = fz + static_cast<float>(k);
-                        const float row = fmaf(zk, s.drow, s.row0);
-                        const float val = bilinear<S>(p, a.pitch, cs, row, nvf);
+                        const float val = bilinear<S>(p, a.pitch, cs, fmaf(zk, s.drow, rowp0), nvp);
                         float w = s.w;
                         if constexpr (KIND == GCT_CYLINDRICAL) {
                             const float nvk = fmaf(zk, s.dnv, s.nv0);
@@ -543,7 +610,7 @@ __global__ __launch_bounds__(256) void bp2d_lds_kernel(BpLaunch a, uint32_t vb)
     float *vp = a.vol + static_cast<size_t>(active ? iy : 0u) * a.nx + (active ? ix : 0u);
     float acc = (active && a.accumulate) ? *vp : 0.0f;
     const float fx = static_cast<float>(ix), fy = static_cast<float>(iy);
-    const float nuf = static_cast<float>(a.nu);
+    const float nup = static_cast<float>(a.nu + 1u);
     const size_t view_elems = static_cast<size_t>(a.rows_pv) * a.pitch;
     const T *base = static_cast<const T *>(a.filt);
     const uint32_t vend = a.first_view + a.num_views;
@@ -573,7 +640,7 @@ __global__ __launch_bounds__(256) void bp2d_lds_kernel(BpLaunch a, uint32_t vb)
                     col = fmaf(atan2f(nu, nw), a.inv_dgamma, r.k[0]);
                     w = r.k[3] / fmaf(nu, nu, nw * nw);
                 }
-                const Sample cs = col_sample(col, nuf);
+                const Sample cs = col_sample(col, nup);
                 const float *rr = rows + j * a.pitch + cs.colbase;
                 const float t0 = rr[0], t1 = rr[1];
                 acc = fmaf(fmaf(cs.alpha, t1 - t0, t0), w, acc);

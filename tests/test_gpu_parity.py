@@ -148,22 +148,48 @@ def test_one_shot_high_low_memory_identical(torch_dev, mode):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("setup", [g.SETUP_F32, g.SETUP_F64], ids=["f32", "f64"])
 @pytest.mark.parametrize("kind", KINDS, ids=g.KIND_NAMES)
-def test_lds_and_direct_kernels_identical(torch_dev, kind):
+def test_lds_and_direct_kernels_identical(torch_dev, kind, setup):
     """The LDS-staged and the direct backprojector read the same taps and run
     the same arithmetic per update, and every voxel's row coordinate is stepped
-    from z index 0, so they agree bit for bit whatever the z unroll."""
+    from z index 0, so they agree bit for bit whatever the z unroll and view
+    block, with FP32 and with FP64 per-column set-up."""
     torch = torch_dev
     wl = W.small_case(kind)
     proj = torch.from_numpy(wl.projections()).cuda()
     out = []
-    for bk, zu in ((g.BP_DIRECT, 8), (g.BP_DIRECT, 4), (g.BP_LDS, 16), (g.BP_LDS, 32)):
+    for bk, zu, vb in ((g.BP_DIRECT, 8, 0), (g.BP_DIRECT, 4, 0), (g.BP_LDS, 16, 2), (g.BP_LDS, 16, 4),
+                       (g.BP_LDS, 32, 1), (g.BP_LDS, 32, 2)):
         if g.is_1d(kind) and zu == 32:
             continue
-        plan = g.Plan(wl.scan, wl.img, bp_kernel=bk, z_unroll=0 if g.is_1d(kind) else zu)
+        plan = g.Plan(wl.scan, wl.img, bp_kernel=bk, z_unroll=0 if g.is_1d(kind) else zu,
+                      view_block=vb if bk == g.BP_LDS and not g.is_1d(kind) else 0, setup_f64=setup)
         v = torch.empty(wl.img.shape, dtype=torch.float32, device="cuda")
         plan.execute(proj, v)
         torch.cuda.synchronize()
         out.append(v.cpu().numpy())
     for o in out[1:]:
         assert np.array_equal(o, out[0])
+
+
+def test_lds_and_direct_kernels_identical_512(torch_dev):
+    """The same identity on the 512^3 C-arm short scan (FP64 per-column
+    set-up, FP32 box corners): the LDS kernel clamps each tap just below the
+    last staged row and column, a full row or column beyond the tile's
+    corners, so the clamp never binds where the two set-ups round apart."""
+    torch = torch_dev
+    wl = W.carm_short_scan_512()
+    proj = wl.projections_device()
+    ref = None
+    for bk, zu, vb in ((g.BP_DIRECT, 8, 0), (g.BP_LDS, 16, 2), (g.BP_LDS, 32, 2)):
+        plan = g.Plan(wl.scan, wl.img, window=wl.window, bp_kernel=bk, z_unroll=zu, view_block=vb)
+        assert plan.info().setup_f64 == 1
+        v = torch.empty(wl.img.shape, dtype=torch.float32, device="cuda")
+        plan.execute(proj, v)
+        torch.cuda.synchronize()
+        plan.close()
+        if ref is None:
+            ref = v
+        else:
+            assert torch.equal(v, ref), f"kernel {bk} z_unroll {zu}: {(v != ref).sum().item()} voxels differ"
