@@ -27,9 +27,13 @@ HIPCC = str(ROCM / "bin" / "hipcc")
 CLANGXX = str(ROCM / "llvm" / "bin" / "clang++")
 
 COMMON = ["-O3", "-std=c++20", "-fPIC", f"-I{ROOT / 'include'}", f"-I{CSRC}", "-Wall", "-Wno-unused-function"]
-# Packed FP32 instructions are disabled in device code.
+# Packed FP32 instructions are disabled in device code.  Multiply-adds are
+# fused only within an expression (-ffp-contract=on): fusing across
+# statements would let the compiler round a kernel's arithmetic differently
+# in each template instance, and the backprojection variants must agree bit
+# for bit; kernels spell out the fmaf they rely on.
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-ffp-contract=fast", "-munsafe-fp-atomics"]
+          "-ffp-contract=on", "-munsafe-fp-atomics"]
 HOST = ["-D__HIP_PLATFORM_AMD__", f"-I{ROCM / 'include'}"]
 
 

@@ -49,15 +49,15 @@ struct Sample {
     float alpha;        // horizontal fraction
 };
 
-// Taps are addressed by padded coordinates (detector coordinate + 1).
-// Clamping one to [0, n + 1] keeps every tap inside the padded buffer; any
-// coordinate outside the detector then lands on zero padding, which
-// reproduces the border-zero texture (nd_array_texture.cuh:43-50).  The
-// clamped coordinate is non-negative, so truncation is its floor and
-// v_fract_f32 gives its fraction exactly.
-__device__ inline Sample col_sample(float col, float nup)
+// Taps are addressed by padded coordinates (detector coordinate + 1), which
+// the per-view records give.  Clamping one to [0, n + 1] keeps every tap
+// inside the padded buffer; any coordinate outside the detector then lands on
+// zero padding, which reproduces the border-zero texture
+// (nd_array_texture.cuh:43-50).  The clamped coordinate is non-negative, so
+// truncation is its floor and v_fract_f32 gives its fraction exactly.
+__device__ inline Sample col_sample(float colp, float nup)
 {
-    const float c = __builtin_amdgcn_fmed3f(col + 1.0f, 0.0f, nup);
+    const float c = __builtin_amdgcn_fmed3f(colp, 0.0f, nup);
     return Sample{static_cast<uint32_t>(c), __builtin_amdgcn_fractf(c)};
 }
 
@@ -92,16 +92,19 @@ __device__ inline float linear(const typename Stor<S>::T *__restrict__ p, uint32
 
 // Per (thread, view) setup of a voxel column, shared by all z voxels.
 struct ColSetup {
-    float col;     // column coordinate (unclamped)
+    float col;     // padded column coordinate (unclamped)
     float w;       // weight (constant along z unless cylindrical)
-    float row0;    // row coordinate of the first z voxel
+    float row0;    // padded row coordinate of the first z voxel
     float drow;    // row step per z voxel
     float rho2;    // cylindrical: in-plane squared distance
     float nv0;     // cylindrical: v numerator of the first z voxel
     float dnv;     // cylindrical: v numerator step
+    double row0d, drowd, nv0d;   // row0, drow, nv0 before rounding (FP64 set-up)
 };
 
-template <int KIND, bool F64>
+// Z = false: the column is set up at z index 0 (fz is ignored), which drops
+// the z terms without changing any result.
+template <int KIND, bool F64, bool Z = true>
 __device__ inline ColSetup setup_column(const BpLaunch &a, uint32_t v, float fx, float fy, float fz, float nuf)
 {
     ColSetup s{};
@@ -112,25 +115,39 @@ __device__ inline ColSetup setup_column(const BpLaunch &a, uint32_t v, float fx,
         const BpRec64 &r = a.rec64[v];
         s.col = static_cast<float>(fma(r.u[2], static_cast<double>(fy), fma(r.u[1], static_cast<double>(fx), r.u[0])));
         s.w = 1.0f;
-        s.row0 = static_cast<float>(fma(r.v[3], static_cast<double>(fz), r.v[0]));
+        s.row0d = Z ? fma(r.v[3], static_cast<double>(fz), r.v[0]) : r.v[0];
+        s.drowd = r.v[3];
+        s.row0 = static_cast<float>(s.row0d);
         s.drow = static_cast<float>(r.v[3]);
     } else if constexpr (KIND == GCT_FLAT) {
         if constexpr (F64) {
             const BpRec64 &r = a.rec64[v];
-            const double X = fx, Y = fy, Z = fz;
-            const double nw = fma(r.w[3], Z, fma(r.w[2], Y, fma(r.w[1], X, r.w[0])));
-            const double nu = fma(r.u[3], Z, fma(r.u[2], Y, fma(r.u[1], X, r.u[0])));
-            const double nv = fma(r.v[3], Z, fma(r.v[2], Y, fma(r.v[1], X, r.v[0])));
+            const double X = fx, Y = fy, W = fz;
+            double nw = fma(r.w[2], Y, fma(r.w[1], X, r.w[0]));
+            double nu = fma(r.u[2], Y, fma(r.u[1], X, r.u[0]));
+            double nv = fma(r.v[2], Y, fma(r.v[1], X, r.v[0]));
+            if constexpr (Z) {
+                nw = fma(r.w[3], W, nw);
+                nu = fma(r.u[3], W, nu);
+                nv = fma(r.v[3], W, nv);
+            }
             const double rd = 1.0 / nw;
             s.col = static_cast<float>(fma(nu, rd, r.k[0]));
             s.w = static_cast<float>(rd * rd);
-            s.row0 = static_cast<float>(fma(nv, rd, r.k[1]));
-            s.drow = static_cast<float>(r.v[3] * rd);
+            s.row0d = fma(nv, rd, r.k[1]);
+            s.drowd = r.v[3] * rd;
+            s.row0 = static_cast<float>(s.row0d);
+            s.drow = static_cast<float>(s.drowd);
         } else {
             const BpRec &r = a.rec[v];
-            const float nw = fmaf(r.w[3], fz, fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0])));
-            const float nu = fmaf(r.u[3], fz, fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0])));
-            const float nv = fmaf(r.v[3], fz, fmaf(r.v[2], fy, fmaf(r.v[1], fx, r.v[0])));
+            float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
+            float nu = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
+            float nv = fmaf(r.v[2], fy, fmaf(r.v[1], fx, r.v[0]));
+            if constexpr (Z) {
+                nw = fmaf(r.w[3], fz, nw);
+                nu = fmaf(r.u[3], fz, nu);
+                nv = fmaf(r.v[3], fz, nv);
+            }
             const float rd = __builtin_amdgcn_rcpf(nw);   // v_rcp_f32 (1 ulp); nw = 0 gives inf as 1 / nw does
             s.col = fmaf(nu, rd, r.k[0]);
             s.w = rd * rd;
@@ -140,24 +157,37 @@ __device__ inline ColSetup setup_column(const BpLaunch &a, uint32_t v, float fx,
     } else if constexpr (KIND == GCT_CYLINDRICAL) {
         if constexpr (F64) {
             const BpRec64 &r = a.rec64[v];
-            const double X = fx, Y = fy, Z = fz;
-            const double nw = fma(r.w[3], Z, fma(r.w[2], Y, fma(r.w[1], X, r.w[0])));
-            const double nu = fma(r.u[3], Z, fma(r.u[2], Y, fma(r.u[1], X, r.u[0])));
-            const double nv = fma(r.v[3], Z, fma(r.v[2], Y, fma(r.v[1], X, r.v[0])));
+            const double X = fx, Y = fy, W = fz;
+            double nw = fma(r.w[2], Y, fma(r.w[1], X, r.w[0]));
+            double nu = fma(r.u[2], Y, fma(r.u[1], X, r.u[0]));
+            double nv = fma(r.v[2], Y, fma(r.v[1], X, r.v[0]));
+            if constexpr (Z) {
+                nw = fma(r.w[3], W, nw);
+                nu = fma(r.u[3], W, nu);
+                nv = fma(r.v[3], W, nv);
+            }
             const double rho2 = fma(nu, nu, nw * nw);
             const double rr = 1.0 / sqrt(rho2);
             s.col = static_cast<float>(fma(atan2(nu, nw), static_cast<double>(a.inv_dgamma), r.k[0]));
-            s.row0 = static_cast<float>(fma(r.k[2] * nv, rr, r.k[1]));
-            s.drow = static_cast<float>(r.k[2] * r.v[3] * rr);
+            s.row0d = fma(r.k[2] * nv, rr, r.k[1]);
+            s.drowd = r.k[2] * r.v[3] * rr;
+            s.row0 = static_cast<float>(s.row0d);
+            s.drow = static_cast<float>(s.drowd);
             s.rho2 = static_cast<float>(rho2);
+            s.nv0d = nv;
             s.nv0 = static_cast<float>(nv);
             s.dnv = static_cast<float>(r.v[3]);
             s.w = static_cast<float>(r.k[3]);
         } else {
             const BpRec &r = a.rec[v];
-            const float nw = fmaf(r.w[3], fz, fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0])));
-            const float nu = fmaf(r.u[3], fz, fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0])));
-            const float nv = fmaf(r.v[3], fz, fmaf(r.v[2], fy, fmaf(r.v[1], fx, r.v[0])));
+            float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
+            float nu = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
+            float nv = fmaf(r.v[2], fy, fmaf(r.v[1], fx, r.v[0]));
+            if constexpr (Z) {
+                nw = fmaf(r.w[3], fz, nw);
+                nu = fmaf(r.u[3], fz, nu);
+                nv = fmaf(r.v[3], fz, nv);
+            }
             const float rho2 = fmaf(nu, nu, nw * nw);
             const float rr = 1.0f / sqrtf(rho2);
             s.col = fmaf(atan2f(nu, nw), a.inv_dgamma, r.k[0]);
@@ -171,6 +201,35 @@ __device__ inline ColSetup setup_column(const BpLaunch &a, uint32_t v, float fx,
     }
     return s;
 }
+
+// Row coordinate (and cylindrical v numerator) of a column at the base zb of
+// a z block: in FP64 when the set-up is (one rounding), else in FP32.
+template <int KIND, bool F64>
+__device__ inline float row_base(const ColSetup &s, float zb)
+{
+    if constexpr (F64 || KIND == GCT_PARALLEL_2D || KIND == GCT_CONE_PARALLEL)
+        return static_cast<float>(fma(static_cast<double>(zb), s.drowd, s.row0d));
+    else
+        return fmaf(zb, s.drow, s.row0);
+}
+
+template <int KIND, bool F64>
+__device__ inline float nv_base(const ColSetup &s, float zb)
+{
+    if constexpr (KIND != GCT_CYLINDRICAL)
+        return 0.0f;
+    else if constexpr (F64)
+        return static_cast<float>(fma(static_cast<double>(zb), static_cast<double>(s.dnv), s.nv0d));
+    else
+        return fmaf(zb, s.dnv, s.nv0);
+}
+
+// A voxel column's row coordinate (and cylindrical v numerator) at slice
+// z = zb + kk, zb a multiple of kZBlock: base = fma(zb, step, value at z 0),
+// then fma(kk, step, base).  Every kernel splits z this way, so a result
+// depends on z alone and not on how z is cut into tiles, and a view costs one
+// fma per kZBlock slices besides one per slice.
+constexpr uint32_t kZBlock = 16;
 
 // Whether the row axis of a view stays parallel to z: then u, w and the
 // weight are constant along a voxel column and only the row coordinate moves.
@@ -201,6 +260,8 @@ __device__ inline void bp_tile3d(const BpLaunch &a, uint32_t bx, uint32_t by, ui
     for (int k = 0; k < ZU; ++k) acc[k] = (a.accumulate && static_cast<uint32_t>(k) < zn) ? vp[k * slice] : 0.0f;
 
     const float fx = static_cast<float>(ix), fy = static_cast<float>(iy), fz = static_cast<float>(iz0);
+    static_assert(kZBlock % ZU == 0, "a direct tile lies within one z block");
+    const float fzb = static_cast<float>(iz0 & ~(kZBlock - 1u)), fko = static_cast<float>(iz0 & (kZBlock - 1u));
     const float nuf = static_cast<float>(a.nu), nup = static_cast<float>(a.nu + 1u);
     const float nvp = static_cast<float>(a.nv + 1u);
     const size_t view_elems = static_cast<size_t>(a.rows_pv) * a.pitch;
@@ -211,16 +272,17 @@ __device__ inline void bp_tile3d(const BpLaunch &a, uint32_t bx, uint32_t by, ui
         if constexpr (column_invariant<KIND, CIRC>()) {
             // set up at z index 0 and step by the absolute index, so the
             // result does not depend on how z is split into tiles
-            const ColSetup s = setup_column<KIND, F64>(a, v, fx, fy, 0.0f, nuf);
+            const ColSetup s = setup_column<KIND, F64, false>(a, v, fx, fy, 0.0f, nuf);
             const Sample cs = col_sample(s.col, nup);
-            const float rowp0 = s.row0 + 1.0f;
+            const float rowb = row_base<KIND, F64>(s, fzb);
+            const float nvb = nv_base<KIND, F64>(s, fzb);
 #pragma unroll
             for (int k = 0; k < ZU; ++k) {
-                const float zk = fz + static_cast<float>(k);
-                const float val = bilinear<S>(p, a.pitch, cs, fmaf(zk, s.drow, rowp0), nvp);
+                const float kk = fko + static_cast<float>(k);
+                const float val = bilinear<S>(p, a.pitch, cs, fmaf(kk, s.drow, rowb), nvp);
                 float w = s.w;
                 if constexpr (KIND == GCT_CYLINDRICAL) {
-                    const float nvk = fmaf(zk, s.dnv, s.nv0);
+                    const float nvk = fmaf(kk, s.dnv, nvb);
                     w = s.w * __builtin_amdgcn_rcpf(fmaf(nvk, nvk, s.rho2));
                 }
                 acc[k] = fmaf(val, w, acc[k]);
@@ -229,7 +291,7 @@ __device__ inline void bp_tile3d(const BpLaunch &a, uint32_t bx, uint32_t by, ui
 #pragma unroll
             for (int k = 0; k < ZU; ++k) {
                 const ColSetup s = setup_column<KIND, F64>(a, v, fx, fy, fz + static_cast<float>(k), nuf);
-                const float val = bilinear<S>(p, a.pitch, col_sample(s.col, nup), s.row0 + 1.0f, nvp);
+                const float val = bilinear<S>(p, a.pitch, col_sample(s.col, nup), s.row0, nvp);
                 float w = s.w;
                 if constexpr (KIND == GCT_CYLINDRICAL) w = s.w * __builtin_amdgcn_rcpf(fmaf(s.nv0, s.nv0, s.rho2));
                 acc[k] = fmaf(val, w, acc[k]);
@@ -332,8 +394,15 @@ __device__ inline float next_lane(float x)
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xF, 0xF, true));
 }
 
-struct Box {
+// A view's staged region.  Box row r, column c is padded pixel
+// (rlo + 1 + r, clo + 1 + c).  The backprojection clamps padded coordinates
+// to [c0, c1] and [r0, r1] (just below the last staged column and row, so
+// both taps of a sample are staged), and the byte offset of pair (r, c) in
+// the tile is 8 (ebase + padded row kTP + padded column).
+struct alignas(16) Box {
     int clo, rlo, ncols, nrows;   // ncols == 0: the view is read from HBM
+    float c0, c1, r0, r1;
+    int ebase, pad0, pad1, pad2;
 };
 
 template <int KIND, int ZU>
@@ -346,7 +415,7 @@ __device__ inline void tile_boxes(const BpLaunch &a, uint32_t bx, uint32_t by, u
     const uint32_t cy = min(by * 16u + ((corner & 2u) ? 15u : 0u), a.ny - 1u);
     const uint32_t cz = min(bz * ZU + ((corner & 4u) ? static_cast<uint32_t>(ZU - 1) : 0u), a.nz - 1u);
     const float nuf = static_cast<float>(a.nu);
-    const ColSetup s = setup_column<KIND, false>(a, vb0 + j, static_cast<float>(cx), static_cast<float>(cy), 0.0f, nuf);
+    const ColSetup s = setup_column<KIND, false, false>(a, vb0 + j, static_cast<float>(cx), static_cast<float>(cy), 0.0f, nuf);
     const float row = fmaf(static_cast<float>(cz), s.drow, s.row0);
     float cmin = s.col, cmax = s.col, rmin = row, rmax = row;
 #pragma unroll
@@ -357,15 +426,27 @@ __device__ inline void tile_boxes(const BpLaunch &a, uint32_t bx, uint32_t by, u
         rmax = fmaxf(rmax, __shfl_xor(rmax, off));
     }
     if (corner != 0u) return;
-    Box b{0, 0, 0, 0};
+    Box b{};
     const bool finite = cmin > -1.0e6f && cmax < 1.0e6f && rmin > -1.0e6f && rmax < 1.0e6f;
     if (finite) {
-        const int clo = max(static_cast<int>(floorf(cmin)) - 1, -1);
-        const int chi = min(static_cast<int>(floorf(cmax)) + 2, static_cast<int>(a.nu) + 1);
-        const int rlo = max(static_cast<int>(floorf(rmin)) - 1, -1);
-        const int rhi = min(static_cast<int>(floorf(rmax)) + 2, static_cast<int>(a.nv) + 1);
+        // corner coordinates are padded (+1); clo .. chi, rlo .. rhi are detector
+        // columns and rows, one beyond the corners' taps on each side
+        const int clo = max(static_cast<int>(floorf(cmin)) - 2, -1);
+        const int chi = min(static_cast<int>(floorf(cmax)) + 1, static_cast<int>(a.nu) + 1);
+        const int rlo = max(static_cast<int>(floorf(rmin)) - 2, -1);
+        const int rhi = min(static_cast<int>(floorf(rmax)) + 1, static_cast<int>(a.nv) + 1);
         const int nc = chi - clo + 1, nr = rhi - rlo + 1;
-        if (nc >= 2 && nr >= 2 && nc - 1 <= kTP && nr <= TR) b = Box{clo, rlo, nc, nr};
+        if (nc >= 2 && nr >= 2 && nc - 1 <= kTP && nr <= TR) {
+            b.clo = clo;
+            b.rlo = rlo;
+            b.ncols = nc;
+            b.nrows = nr;
+            b.c0 = static_cast<float>(clo + 1);
+            b.c1 = below(static_cast<float>(clo + nc));
+            b.r0 = static_cast<float>(rlo + 1);
+            b.r1 = below(static_cast<float>(rlo + nr));
+            b.ebase = static_cast<int>(j) * (TR * kTP) - (clo + 1) - (rlo + 1) * kTP;
+        }
     }
     boxes[j] = b;
 }
@@ -468,6 +549,8 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
         acc[k] = (active && a.accumulate && static_cast<uint32_t>(k) < zn) ? vp[k * slice] : 0.0f;
 
     const float fx = static_cast<float>(ix), fy = static_cast<float>(iy), fz = static_cast<float>(iz0);
+    static_assert(ZU % kZBlock == 0, "an LDS tile is whole z blocks");
+    constexpr int kZB = ZU / static_cast<int>(kZBlock);
     const float nuf = static_cast<float>(a.nu), nup = static_cast<float>(a.nu + 1u);
     const float nvp = static_cast<float>(a.nv + 1u);
     const size_t view_elems = static_cast<size_t>(a.rows_pv) * a.pitch;
@@ -501,32 +584,35 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
         if (active) {
             for (uint32_t j = 0; j < nvb; ++j) {
                 const uint32_t v = vb0 + j;
-                const ColSetup s = setup_column<KIND, F64>(a, v, fx, fy, 0.0f, nuf);
+                const ColSetup s = setup_column<KIND, F64, false>(a, v, fx, fy, 0.0f, nuf);
                 const Box b = boxes[j];
                 if (b.ncols != 0) {
-                    // Padded coordinates, as in col_sample / bilinear.  The box holds padded
-                    // columns clo + 1 .. clo + ncols and rows rlo + 1 .. rlo + nrows; clamping
-                    // below the last of each keeps both taps inside it, and never binds for a
-                    // coordinate of this tile, which the box covers with a margin.
-                    const float clp = __builtin_amdgcn_fmed3f(s.col + 1.0f, static_cast<float>(b.clo + 1),
-                                                              below(static_cast<float>(b.clo + b.ncols)));
+                    // Padded coordinates, as in col_sample / bilinear.  The box clamps never
+                    // bind for a coordinate of this tile, which the box covers with a margin.
+                    const float clp = __builtin_amdgcn_fmed3f(s.col, b.c0, b.c1);
                     const float alpha = __builtin_amdgcn_fractf(clp);
-                    // Pair (r, c) of the box sits at byte 8 (j TR kTP + r kTP + c).  The per-view
-                    // terms fold into one base (modulo 2^32: it is negative when the box starts
-                    // below row 0 of view j's slot); the padded row index is added with one
-                    // 24-bit multiply-add.
-                    const uint32_t obase = 8u * static_cast<uint32_t>(static_cast<int>(j) * (TR * kTP) +
-                                                                      (static_cast<int>(clp) - (b.clo + 1)) -
-                                                                      (b.rlo + 1) * kTP);
-                    const float rlo = static_cast<float>(b.rlo + 1), rhi = below(static_cast<float>(b.rlo + b.nrows));
-                    const float rowp0 = s.row0 + 1.0f;
+                    // The per-view terms of the byte offset fold into one base (modulo 2^32:
+                    // it is negative when the box starts below row 0 of view j's slot); the
+                    // padded row index is added with one 24-bit multiply-add.
+                    const uint32_t obase = 8u * static_cast<uint32_t>(b.ebase + static_cast<int>(clp));
+                    const float rlo = b.r0, rhi = b.r1;
+                    float rowb[kZB], nvb[kZB];
+#pragma unroll
+                    for (int g = 0; g < kZB; ++g) {
+                        const float zb = fz + static_cast<float>(g * static_cast<int>(kZBlock));
+                        rowb[g] = row_base<KIND, F64>(s, zb);
+                        nvb[g] = nv_base<KIND, F64>(s, zb);
+                    }
 #pragma unroll
                     for (int k = 0; k < ZU; ++k) {
                         if constexpr (WIN) {
                             if (v < wl[k] || v >= wh[k]) continue;
                         }
-                        const float zk = fz + static_cast<float>(k);
-                        const float row = __builtin_amdgcn_fmed3f(fmaf(zk, s.drow, rowp0), rlo, rhi);
+                        const float kk = static_cast<float>(k % static_cast<int>(kZBlock));
+                        const int g = k / static_cast<int>(kZBlock);
+                        // fma(0, drow, base) is base: slice 0 of a block takes the base as is
+                        const float rk = k % static_cast<int>(kZBlock) == 0 ? rowb[g] : fmaf(kk, s.drow, rowb[g]);
+                        const float row = __builtin_amdgcn_fmed3f(rk, rlo, rhi);
                         const float beta = __builtin_amdgcn_fractf(row);
                         const uint32_t off = __umul24(static_cast<uint32_t>(row), 8u * kTP) + obase;
                         const float2 *q = reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(tile) + off);
@@ -536,7 +622,7 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                         const float val = fmaf(beta, bot - top, top);
                         float w = s.w;
                         if constexpr (KIND == GCT_CYLINDRICAL) {
-                            const float nvk = fmaf(zk, s.dnv, s.nv0);
+                            const float nvk = fmaf(kk, s.dnv, nvb[g]);
                             w = s.w * __builtin_amdgcn_rcpf(fmaf(nvk, nvk, s.rho2));
                         }
                         acc[k] = fmaf(val, w, acc[k]);
@@ -544,17 +630,24 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                 } else {
                     const T *p = base + static_cast<size_t>(v) * view_elems;
                     const Sample cs = col_sample(s.col, nup);
-                    const float rowp0 = s.row0 + 1.0f;
+                    float rowb[kZB], nvb[kZB];
+#pragma unroll
+                    for (int g = 0; g < kZB; ++g) {
+                        const float zb = fz + static_cast<float>(g * static_cast<int>(kZBlock));
+                        rowb[g] = row_base<KIND, F64>(s, zb);
+                        nvb[g] = nv_base<KIND, F64>(s, zb);
+                    }
 #pragma unroll
                     for (int k = 0; k < ZU; ++k) {
                         if constexpr (WIN) {
                             if (v < wl[k] || v >= wh[k]) continue;
                         }
-                        const float zk = fz + static_cast<float>(k);
-                        const float val = bilinear<S>(p, a.pitch, cs, fmaf(zk, s.drow, rowp0), nvp);
+                        const float kk = static_cast<float>(k % static_cast<int>(kZBlock));
+                        const int g = k / static_cast<int>(kZBlock);
+                        const float val = bilinear<S>(p, a.pitch, cs, fmaf(kk, s.drow, rowb[g]), nvp);
                         float w = s.w;
                         if constexpr (KIND == GCT_CYLINDRICAL) {
-                            const float nvk = fmaf(zk, s.dnv, s.nv0);
+                            const float nvk = fmaf(kk, s.dnv, nvb[g]);
                             w = s.w * __builtin_amdgcn_rcpf(fmaf(nvk, nvk, s.rho2));
                         }
                         acc[k] = fmaf(val, w, acc[k]);

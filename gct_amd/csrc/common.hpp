@@ -84,7 +84,9 @@ uint32_t next_pow2(uint32_t n);
 
 // Per-view backprojection record: 16 floats, read with one scalar load.
 //   u[4], v[4], w[4]: affine forms in voxel indices (1, ix, iy, iz)
-//   k[4]: kind-specific constants (see plan.cpp: build_bp_records)
+//   k[4]: kind-specific constants (see plan.cpp: build_records)
+// Detector coordinates derived from a record are padded ones (column + 1,
+// row + 1), the indices of the filtered views' zero-bordered layout.
 struct alignas(64) BpRec {
     float u[4];
     float v[4];

@@ -98,7 +98,12 @@ void build_records(gct_plan &p, std::vector<BpRec> &rec, std::vector<BpRec64> &r
     for (uint32_t v = 0; v < p.V; ++v) {
         const gct_proj_geom &pg = p.views[v];
         BpRec64 r{};
-        const double cu = pg.det_row_center, cv = pg.det_col_center, D = pg.src_det_dist;
+        // The records give padded detector coordinates (column + 1, row + 1):
+        // a filtered view carries one zero column and row before its first, and
+        // the backprojection addresses its taps from there.  Adding the 1 to the
+        // centre here, in FP64, keeps it from costing a rounding per column.
+        const double cu = double(pg.det_row_center) + 1.0, cv = double(pg.det_col_center) + 1.0;
+        const double D = pg.src_det_dist;
         double s[3] = {pg.src_pos[0], pg.src_pos[1], is_1d(g.kind) ? 0.0 : pg.src_pos[2]};
         double eu[3] = {pg.det_row_dir[0], pg.det_row_dir[1], is_1d(g.kind) ? 0.0 : pg.det_row_dir[2]};
         double ev[3] = {pg.det_col_dir[0], pg.det_col_dir[1], pg.det_col_dir[2]};
