@@ -170,6 +170,7 @@ def run_flat(a, g, W, torch, np, sections):
             "storage": "f32",
             "bp_kernel": variant,
             "z_unroll": info.z_unroll,
+            "view_block": info.view_block,
             "autotune_best_bp_ms": round(tuned_ms, 4),
             "fft_len": info.fft_len,
         },
@@ -236,6 +237,7 @@ def run_precision(a, g, W, torch, np):
         v = torch.empty(wl.img.shape, dtype=torch.float32, device="cuda")
         plan.execute(p, v, stream=stream)
         torch.cuda.synchronize()
+        plan.autotune(v)          # each storage form picks its own backprojection variant
         ts = []
         for _ in range(3):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -248,6 +250,8 @@ def run_precision(a, g, W, torch, np):
         info = plan.info()
         d = v - truth
         f = {"ms": round(ms, 3), "gups": round(wl.updates / (ms * 1e-3) / 1e9, 2),
+             "bp_variant": {"kernel": "lds" if info.bp_kernel == g.BP_LDS else "direct",
+                            "z_unroll": int(info.z_unroll), "view_block": int(info.view_block)},
              "filtered_bytes": int(info.filt_bytes), "setup_f64": bool(info.setup_f64),
              "rmse_vs_phantom": float(torch.sqrt(torch.mean(d * d)))}
         if ref is None:
