@@ -170,7 +170,7 @@ def run_flat(a, g, W, torch, np, sections):
             "storage": "f32",
             "bp_kernel": variant,
             "z_unroll": info.z_unroll,
-            "view_block": info.view_block,
+            "lds_shape": {"view_block": info.view_block, "rows": info.lds_rows, "pairs": info.lds_pairs},
             "autotune_best_bp_ms": round(tuned_ms, 4),
             "fft_len": info.fft_len,
         },
@@ -251,7 +251,8 @@ def run_precision(a, g, W, torch, np):
         d = v - truth
         f = {"ms": round(ms, 3), "gups": round(wl.updates / (ms * 1e-3) / 1e9, 2),
              "bp_variant": {"kernel": "lds" if info.bp_kernel == g.BP_LDS else "direct",
-                            "z_unroll": int(info.z_unroll), "view_block": int(info.view_block)},
+                            "z_unroll": int(info.z_unroll), "view_block": int(info.view_block),
+                            "rows": int(info.lds_rows), "pairs": int(info.lds_pairs)},
              "filtered_bytes": int(info.filt_bytes), "setup_f64": bool(info.setup_f64),
              "rmse_vs_phantom": float(torch.sqrt(torch.mean(d * d)))}
         if ref is None:

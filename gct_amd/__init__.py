@@ -73,7 +73,7 @@ class ReconOpts(C.Structure):
     _fields_ = [("cut_off_freq", C.c_float), ("window", C.c_int32), ("storage", C.c_int32),
                 ("setup_precision", C.c_int32), ("mem_mode", C.c_int32), ("device", C.c_int32),
                 ("z_unroll", C.c_uint32), ("view_chunk", C.c_uint32), ("bp_kernel", C.c_int32),
-                ("view_block", C.c_uint32)]
+                ("view_block", C.c_uint32), ("lds_rows", C.c_uint32), ("lds_pairs", C.c_uint32)]
 
 
 class CircularParams(C.Structure):
@@ -87,7 +87,8 @@ class PlanInfo(C.Structure):
                 ("z_unroll", C.c_uint32), ("filt_bytes", C.c_uint64), ("device_bytes", C.c_uint64),
                 ("updates", C.c_uint64), ("rot_ang_incr", C.c_float), ("rot_ang_range", C.c_float),
                 ("rot_dir", C.c_int32), ("setup_f64", C.c_int32), ("circular", C.c_int32), ("storage", C.c_int32),
-                ("bp_kernel", C.c_int32), ("view_block", C.c_uint32)]
+                ("bp_kernel", C.c_int32), ("view_block", C.c_uint32), ("lds_rows", C.c_uint32),
+                ("lds_pairs", C.c_uint32)]
 
 
 class ServerOpts(C.Structure):

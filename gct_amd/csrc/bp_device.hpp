@@ -385,7 +385,6 @@ __global__ __launch_bounds__(64) void bp2d_kernel(BpLaunch a)
 // clamp also reproduces the border-zero texture, and it keeps every LDS read
 // inside the box.  A view whose box does not fit is read from HBM directly.
 
-constexpr int kTP = 48;   // staged (value, difference) pairs per row
 
 // Lane n + 1's value (a wavefront shift by one lane, one DPP move; lane 63
 // reads zero).
@@ -398,7 +397,8 @@ __device__ inline float next_lane(float x)
 // (rlo + 1 + r, clo + 1 + c).  The backprojection clamps padded coordinates
 // to [c0, c1] and [r0, r1] (just below the last staged column and row, so
 // both taps of a sample are staged), and the byte offset of pair (r, c) in
-// the tile is 8 (ebase + padded row kTP + padded column).
+// the tile is 8 (ebase + padded row TP + padded column), TP being the
+// staging area's pairs per row.
 struct alignas(16) Box {
     int clo, rlo, ncols, nrows;   // ncols == 0: the view is read from HBM
     float c0, c1, r0, r1;
@@ -407,7 +407,7 @@ struct alignas(16) Box {
 
 template <int KIND, int ZU>
 __device__ inline void tile_boxes(const BpLaunch &a, uint32_t bx, uint32_t by, uint32_t bz, uint32_t vb0,
-                                  uint32_t nvb, int TR, Box *boxes, uint32_t tid)
+                                  uint32_t nvb, int TR, int TP, Box *boxes, uint32_t tid)
 {
     if (tid >= 8u * nvb) return;
     const uint32_t j = tid >> 3, corner = tid & 7u;
@@ -436,7 +436,7 @@ __device__ inline void tile_boxes(const BpLaunch &a, uint32_t bx, uint32_t by, u
         const int rlo = max(static_cast<int>(floorf(rmin)) - 2, -1);
         const int rhi = min(static_cast<int>(floorf(rmax)) + 1, static_cast<int>(a.nv) + 1);
         const int nc = chi - clo + 1, nr = rhi - rlo + 1;
-        if (nc >= 2 && nr >= 2 && nc - 1 <= kTP && nr <= TR) {
+        if (nc >= 2 && nr >= 2 && nc - 1 <= TP && nr <= TR) {
             b.clo = clo;
             b.rlo = rlo;
             b.ncols = nc;
@@ -445,7 +445,7 @@ __device__ inline void tile_boxes(const BpLaunch &a, uint32_t bx, uint32_t by, u
             b.c1 = below(static_cast<float>(clo + nc));
             b.r0 = static_cast<float>(rlo + 1);
             b.r1 = below(static_cast<float>(rlo + nr));
-            b.ebase = static_cast<int>(j) * (TR * kTP) - (clo + 1) - (rlo + 1) * kTP;
+            b.ebase = static_cast<int>(j) * (TR * TP) - (clo + 1) - (rlo + 1) * TP;
         }
     }
     boxes[j] = b;
@@ -477,18 +477,18 @@ __device__ inline void stage_load(const typename Stor<S>::T *p, uint32_t pitch, 
     }
 }
 
-template <int RSTEP, int NIT, bool HALF>
+template <int TP, int RSTEP, int NIT, bool HALF>
 __device__ inline void stage_store(float2 *tl, int rw, int m, int nrows, uint32_t lane, const float *v)
 {
     const int c = HALF ? static_cast<int>(lane & 31u) : static_cast<int>(lane);
     const int h = HALF ? static_cast<int>(lane >> 5) : 0;
-    float2 *tw = tl + (rw + h) * kTP + c;
+    float2 *tw = tl + (rw + h) * TP + c;
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
         const int r = rw + it * RSTEP;
         if (r < nrows) {
             const float d = next_lane(v[it]) - v[it];
-            if (c < m && r + h < nrows) tw[it * RSTEP * kTP] = make_float2(v[it], d);
+            if (c < m && r + h < nrows) tw[it * RSTEP * TP] = make_float2(v[it], d);
         }
     }
 }
@@ -497,7 +497,7 @@ __device__ inline void stage_store(float2 *tl, int rw, int m, int nrows, uint32_
 // (all loads of a view are issued before its first store).  Box row r,
 // column c is padded pixel (rlo + 1 + r, clo + 1 + c); a box of fewer than
 // 32 pairs per row is staged two rows per wave.
-template <int S, int TR, int VB>
+template <int S, int TR, int VB, int TP>
 __device__ inline void stage_block(const typename Stor<S>::T *base, size_t view_elems, uint32_t pitch, uint32_t vb0,
                                    uint32_t nvb, const Box *bx, float2 *tile, int wave, uint32_t lane)
 {
@@ -509,30 +509,30 @@ __device__ inline void stage_block(const typename Stor<S>::T *base, size_t view_
         const int nrows = __builtin_amdgcn_readfirstlane(bx[j].nrows);
         const typename Stor<S>::T *p = base + static_cast<size_t>(vb0 + j) * view_elems +
                                        (static_cast<size_t>(rlo + 1) * pitch + static_cast<uint32_t>(clo + 1));
-        float2 *tl = tile + j * (TR * kTP);
+        float2 *tl = tile + j * (TR * TP);
         const int m = ncols - 1;
         if (m < 32) {
             constexpr int kN = (TR + 7) / 8;
             float v[kN];
             stage_load<S, 8, kN, true>(p, pitch, 2 * wave, m, nrows, lane, v);
-            stage_store<8, kN, true>(tl, 2 * wave, m, nrows, lane, v);
+            stage_store<TP, 8, kN, true>(tl, 2 * wave, m, nrows, lane, v);
         } else {
             constexpr int kW = (TR + 3) / 4;
             float v[kW];
             stage_load<S, 4, kW, false>(p, pitch, wave, m, nrows, lane, v);
-            stage_store<4, kW, false>(tl, wave, m, nrows, lane, v);
+            stage_store<TP, 4, kW, false>(tl, wave, m, nrows, lane, v);
         }
     }
 }
 
 // One 16 x 16 x ZU tile (bx, by, bz) of a circular 3-D scan; every thread of
 // the 256-thread workgroup calls it (it holds workgroup barriers).  `tile`
-// and `boxes` are the workgroup's LDS staging area of VB * TR * kTP pairs and
-// VB boxes.
+// and `boxes` are the workgroup's LDS staging area of VB * TR * TP pairs
+// (TR rows of TP pairs per view) and VB boxes.
 // WIN: spiral scans -- slice z only takes the views [zwin[2z], zwin[2z+1]);
 // the tile's view loop covers the union of its slices' windows, and the
 // per-slice test is on wave-uniform values (scalar branches).
-template <int KIND, int S, bool F64, int ZU, int TR, int VB, bool WIN = false>
+template <int KIND, int S, bool F64, int ZU, int TR, int VB, int TP, bool WIN = false>
 __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint32_t bz, uint32_t tid, float2 *tile,
                               Box *boxes)
 {
@@ -577,9 +577,9 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
     const uint32_t lane = tid & 63u;
     for (uint32_t vb0 = vfirst; vb0 < vend; vb0 += VB) {
         const uint32_t nvb = min(static_cast<uint32_t>(VB), vend - vb0);
-        tile_boxes<KIND, ZU>(a, bx, by, bz, vb0, nvb, TR, boxes, tid);
+        tile_boxes<KIND, ZU>(a, bx, by, bz, vb0, nvb, TR, TP, boxes, tid);
         __syncthreads();
-        stage_block<S, TR, VB>(base, view_elems, a.pitch, vb0, nvb, boxes, tile, wave, lane);
+        stage_block<S, TR, VB, TP>(base, view_elems, a.pitch, vb0, nvb, boxes, tile, wave, lane);
         __syncthreads();
         if (active) {
             for (uint32_t j = 0; j < nvb; ++j) {
@@ -614,9 +614,9 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                         const float rk = k % static_cast<int>(kZBlock) == 0 ? rowb[g] : fmaf(kk, s.drow, rowb[g]);
                         const float row = __builtin_amdgcn_fmed3f(rk, rlo, rhi);
                         const float beta = __builtin_amdgcn_fractf(row);
-                        const uint32_t off = __umul24(static_cast<uint32_t>(row), 8u * kTP) + obase;
+                        const uint32_t off = __umul24(static_cast<uint32_t>(row), 8u * TP) + obase;
                         const float2 *q = reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(tile) + off);
-                        const float2 tp = q[0], bt = q[kTP];
+                        const float2 tp = q[0], bt = q[TP];
                         const float top = fmaf(alpha, tp.y, tp.x);
                         const float bot = fmaf(alpha, bt.y, bt.x);
                         const float val = fmaf(beta, bot - top, top);
@@ -666,26 +666,29 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
 
 // Occupancy target of an LDS kernel variant: workgroups (one wave per SIMD
 // each) that fit the 160 KiB of LDS per CU, capped by what its registers
-// allow without spilling (z-unroll 16: 5, i.e. <= 96 VGPRs; 32: 3).
-template <int ZU, int TR, int VB>
+// reach (z-unroll 16: 7, i.e. <= 72 VGPRs, which the compiler meets with a
+// few spills; 32: 3).  More resident workgroups hide more of the staging
+// latency: measured, the 6- and 7-workgroup shapes beat the 5-workgroup one
+// despite their spills.
+template <int ZU, int TR, int VB, int TP>
 constexpr int lds_waves_per_simd()
 {
-    constexpr int bytes = VB * TR * kTP * 8 + VB * static_cast<int>(sizeof(Box));
+    constexpr int bytes = VB * TR * TP * 8 + VB * static_cast<int>(sizeof(Box));
     constexpr int by_lds = (160 * 1024) / bytes;
-    constexpr int by_regs = ZU >= 32 ? 3 : 5;
+    constexpr int by_regs = ZU >= 32 ? 3 : 7;
     return by_lds < 1 ? 1 : (by_lds < by_regs ? by_lds : by_regs);
 }
 
-template <int KIND, int S, bool F64, int ZU, int TR, int VB, bool WIN = false>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lds_waves_per_simd<ZU, TR, VB>(), 8)))
+template <int KIND, int S, bool F64, int ZU, int TR, int VB, int TP, bool WIN = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lds_waves_per_simd<ZU, TR, VB, TP>(), 8)))
 void bp3d_lds_kernel(BpLaunch a)
 {
-    __shared__ float2 tile[VB * TR * kTP];
+    __shared__ float2 tile[VB * TR * TP];
     __shared__ Box boxes[VB];
     const uint32_t nbx = (a.nx + 15u) / 16u, nby = (a.ny + 15u) / 16u;
     const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t bx = t % nbx, rest = t / nbx;
-    bp_tile3d_lds<KIND, S, F64, ZU, TR, VB, WIN>(a, bx, rest % nby, rest / nby, threadIdx.x, tile, boxes);
+    bp_tile3d_lds<KIND, S, F64, ZU, TR, VB, TP, WIN>(a, bx, rest % nby, rest / nby, threadIdx.x, tile, boxes);
 }
 
 // 2-D images: each block of views has its (padded) detector rows staged whole
@@ -747,7 +750,10 @@ __global__ __launch_bounds__(256) void bp2d_lds_kernel(BpLaunch a, uint32_t vb)
 // ---- persistent batched backprojection -------------------------------------
 
 constexpr int kBatchZU = static_cast<int>(kBatchDirectZU);   // direct tiles (general trajectories)
-constexpr int kBatchTR = 40, kBatchVB = 2;                    // LDS tiles (circular), depth kBatchLdsZU
+constexpr int kBatchVB = static_cast<int>(kLdsShapes[0].vb);   // LDS tiles (circular): the default shape
+constexpr int kBatchTR = static_cast<int>(kLdsShapes[0].rows);
+constexpr int kBatchTP = static_cast<int>(kLdsShapes[0].pairs);
+static_assert(kLdsShapes[0].zu == kBatchLdsZU, "batched LDS tiles use the default z-unroll-16 shape");
 
 // Persistent backprojection over the scans of one group (same geometry kind,
 // storage and tile type; a batch launches one such kernel per group).  Work
@@ -756,13 +762,14 @@ constexpr int kBatchTR = 40, kBatchVB = 2;                    // LDS tiles (circ
 // drains on its own whatever the mix of scan sizes.
 template <int KIND, int S, bool LDS>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(LDS ? lds_waves_per_simd<static_cast<int>(kBatchLdsZU), kBatchTR, kBatchVB>() : 1, 8)))
+__attribute__((amdgpu_waves_per_eu(LDS ? lds_waves_per_simd<static_cast<int>(kBatchLdsZU), kBatchTR, kBatchVB, kBatchTP>()
+                                       : 1, 8)))
 void bp_batch_kernel(const BatchScan *__restrict__ scans, uint32_t nscans,
                                                        uint32_t total, uint32_t *counter)
 {
     constexpr bool k1d = KIND <= GCT_FAN_LINE_1D;
     __shared__ uint32_t item;
-    __shared__ float2 tile[LDS ? kBatchVB * kBatchTR * kTP : 1];
+    __shared__ float2 tile[LDS ? kBatchVB * kBatchTR * kBatchTP : 1];
     __shared__ Box boxes[kBatchVB];
     const uint32_t tid = threadIdx.x;
     for (;;) {
@@ -786,8 +793,8 @@ void bp_batch_kernel(const BatchScan *__restrict__ scans, uint32_t nscans,
             const uint32_t bx = local % sc.tiles_x, rest = local / sc.tiles_x;
             const uint32_t by = rest % sc.tiles_y, bz = rest / sc.tiles_y;
             if constexpr (LDS)
-                bp_tile3d_lds<KIND, S, false, static_cast<int>(kBatchLdsZU), kBatchTR, kBatchVB>(a, bx, by, bz, tid,
-                                                                                                 tile, boxes);
+                bp_tile3d_lds<KIND, S, false, static_cast<int>(kBatchLdsZU), kBatchTR, kBatchVB, kBatchTP>(
+                    a, bx, by, bz, tid, tile, boxes);
             else
                 bp_tile3d<KIND, S, false, false, kBatchZU>(a, bx, by, bz, tid);
         }
@@ -814,22 +821,29 @@ template <int KIND, int S, bool F64>
 void launch_bp3d_lds(const BpLaunch &a, hipStream_t s)
 {
     const uint32_t nbx = (a.nx + 15u) / 16u, nby = (a.ny + 15u) / 16u;
-    const uint32_t zu = (a.z_unroll == 32u) ? 32u : 16u;
-    const uint32_t blocks = nbx * nby * ((a.nz + zu - 1u) / zu);
-    const dim3 g(blocks), b(256);
+    const dim3 b(256);
     if constexpr (KIND == GCT_FLAT || KIND == GCT_CYLINDRICAL) {
-        if (a.zwin) {   // spiral: per-slice view windows, default variant
-            hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 2, true>),
+        if (a.zwin) {   // spiral: per-slice view windows, default shape
+            hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 2, 40, true>),
                                dim3(nbx * nby * ((a.nz + 15u) / 16u)), b, 0, s, a);
             return;
         }
     }
-    switch (zu * 8u + lds_view_block(zu, a.view_block)) {
-    case 32u * 8u + 1u: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 32, 64, 1>), g, b, 0, s, a); break;
-    case 32u * 8u + 2u: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 32, 64, 2>), g, b, 0, s, a); break;
-    case 16u * 8u + 3u: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 3>), g, b, 0, s, a); break;
-    case 16u * 8u + 4u: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 4>), g, b, 0, s, a); break;
-    default: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 2>), g, b, 0, s, a); break;
+    // the plan resolved (z_unroll, view_block, lds_rows, lds_pairs) to a row of kLdsShapes
+    const int shape = lds_shape(a.z_unroll, a.view_block, a.lds_rows, a.lds_pairs);
+    const uint32_t zu = shape >= 0 ? kLdsShapes[shape].zu : 16u;
+    const dim3 g(nbx * nby * ((a.nz + zu - 1u) / zu));
+    static_assert(kLdsShapes[0].zu == 16 && kLdsShapes[0].vb == 2 && kLdsShapes[0].rows == 40 && kLdsShapes[0].pairs == 40);
+    static_assert(kLdsShapes[1].zu == 16 && kLdsShapes[1].vb == 2 && kLdsShapes[1].rows == 32 && kLdsShapes[1].pairs == 40);
+    static_assert(kLdsShapes[2].zu == 16 && kLdsShapes[2].vb == 2 && kLdsShapes[2].rows == 40 && kLdsShapes[2].pairs == 48);
+    static_assert(kLdsShapes[3].zu == 32 && kLdsShapes[3].vb == 1 && kLdsShapes[3].rows == 64 && kLdsShapes[3].pairs == 40);
+    static_assert(kLdsShapes[4].zu == 32 && kLdsShapes[4].vb == 2 && kLdsShapes[4].rows == 64 && kLdsShapes[4].pairs == 48);
+    switch (shape) {
+    case 1: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 32, 2, 40>), g, b, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 2, 48>), g, b, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 32, 64, 1, 40>), g, b, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 32, 64, 2, 48>), g, b, 0, s, a); break;
+    default: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 2, 40>), g, b, 0, s, a); break;
     }
 }
 

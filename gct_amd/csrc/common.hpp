@@ -143,19 +143,35 @@ struct BpLaunch {
     bool accumulate, finish;
     uint32_t z_unroll;
     int variant;                  // GCT_BP_DIRECT or GCT_BP_LDS
-    uint32_t view_block;          // LDS kernel: views staged per block (0: default for z_unroll)
+    uint32_t view_block;          // LDS kernel shape (a row of kLdsShapes, resolved by the plan):
+    uint32_t lds_rows, lds_pairs; //   views per staged block, staged rows, pairs per staged row
     const uint32_t *zwin;         // spiral: [first, last) view window per z slice (device); null otherwise
 };
 
-// LDS kernel view-block depths per z-unroll: {16: 2, 3, 4 (default 2)}, {32: 1, 2 (default 2)}.
-inline uint32_t lds_view_block(uint32_t z_unroll, uint32_t vb)
+// Shapes of the LDS backprojection kernel: voxels per thread along z, views
+// staged per block, staged rows per view and (value, difference) pairs per
+// staged row.  The LDS a workgroup stages (8 B per pair) sets how many
+// workgroups share a CU: 6 for the first shape, 7 for the second, 5 for the
+// third (for boxes up to 48 pairs wide), 3 for the z-unroll-32 shapes.  The
+// first shape of each z unroll is its default; the autotuner times them all.
+struct LdsShape {
+    uint32_t zu, vb, rows, pairs;
+};
+constexpr LdsShape kLdsShapes[] = {{16, 2, 40, 40}, {16, 2, 32, 40}, {16, 2, 40, 48}, {32, 1, 64, 40}, {32, 2, 64, 48}};
+constexpr int kNumLdsShapes = static_cast<int>(sizeof(kLdsShapes) / sizeof(kLdsShapes[0]));
+
+// The first shape of z unroll `zu` (0: 16) that agrees with every nonzero
+// field; -1 when none does.
+inline int lds_shape(uint32_t zu, uint32_t vb, uint32_t rows, uint32_t pairs)
 {
-    if (z_unroll == 32u) return (vb == 1u) ? 1u : 2u;
-    return (vb == 3u || vb == 4u) ? vb : 2u;
-}
-inline bool lds_view_block_valid(uint32_t z_unroll, uint32_t vb)
-{
-    return vb == 0u || (z_unroll == 32u ? (vb == 1u || vb == 2u) : (vb >= 2u && vb <= 4u));
+    if (zu == 0u) zu = 16u;
+    for (int i = 0; i < kNumLdsShapes; ++i) {
+        const LdsShape &sh = kLdsShapes[i];
+        if (sh.zu == zu && (vb == 0u || vb == sh.vb) && (rows == 0u || rows == sh.rows) &&
+            (pairs == 0u || pairs == sh.pairs))
+            return i;
+    }
+    return -1;
 }
 
 struct FpLaunch {

@@ -61,7 +61,7 @@ void fft64(std::vector<std::complex<double>> &a)
 struct Tuning {
     uint32_t z_unroll;
     int variant;
-    uint32_t view_block;
+    uint32_t view_block, lds_rows, lds_pairs;
 };
 std::mutex g_tune_mu;
 std::map<std::tuple<int, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, int, int, int>, Tuning> g_tune;
@@ -221,8 +221,11 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
     if (o.z_unroll != 0 && !(variant == GCT_BP_DIRECT ? (o.z_unroll == 4 || o.z_unroll == 8 || o.z_unroll == 16)
                                                       : (o.z_unroll == 16 || o.z_unroll == 32)))
         fail(GCT_ERR_INVALID_ARGUMENT, "z_unroll must be 4, 8 or 16 (direct kernel) or 16 or 32 (LDS kernel)");
-    if (!lds_view_block_valid(o.z_unroll ? o.z_unroll : 16u, o.view_block))
-        fail(GCT_ERR_INVALID_ARGUMENT, "view_block must be 2, 3 or 4 (z_unroll 16) or 1 or 2 (z_unroll 32)");
+    if ((o.view_block || o.lds_rows || o.lds_pairs) &&
+        lds_shape(o.z_unroll, o.view_block, o.lds_rows, o.lds_pairs) < 0)
+        fail(GCT_ERR_INVALID_ARGUMENT,
+             "no LDS backprojection shape matches (z_unroll, view_block, lds_rows, lds_pairs); the shapes are "
+             "(16, 2, 40, 40), (16, 2, 32, 40), (16, 2, 40, 48), (32, 1, 64, 40), (32, 2, 64, 48)");
     auto p = std::make_unique<gct_plan>();
     p->device = o.device;
     p->stream = stream;
@@ -324,6 +327,8 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
     p->variant = variant;
     p->z_unroll = o.z_unroll ? o.z_unroll : (variant == GCT_BP_LDS ? 16u : 8u);
     p->view_block = o.view_block;
+    p->lds_rows = o.lds_rows;
+    p->lds_pairs = o.lds_pairs;
     if (!o.z_unroll && o.bp_kernel == GCT_BP_AUTO) {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find({geom.kind, img.x.num_samples, img.y.num_samples, img.dim == 3 ? img.z.num_samples : 1u,
@@ -332,6 +337,8 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
             p->z_unroll = it->second.z_unroll;
             p->variant = it->second.variant;
             p->view_block = it->second.view_block;
+            p->lds_rows = it->second.lds_rows;
+            p->lds_pairs = it->second.lds_pairs;
         }
     }
     return p;
@@ -396,6 +403,8 @@ BpLaunch bp_args(gct_plan &p, float *vol, uint32_t first, uint32_t n, bool accum
     a.z_unroll = p.z_unroll;
     a.variant = p.variant;
     a.view_block = p.view_block;
+    a.lds_rows = p.lds_rows;
+    a.lds_pairs = p.lds_pairs;
     return a;
 }
 
@@ -543,7 +552,11 @@ extern "C" gct_status gct_plan_get_info(const gct_plan *p, gct_plan_info *info)
         info->circular = p->circular;
         info->storage = p->opts.storage;
         info->bp_kernel = p->variant;
-        info->view_block = p->variant == GCT_BP_LDS ? lds_view_block(p->z_unroll, p->view_block) : 0u;
+        const int shape = lds_shape(p->z_unroll, p->view_block, p->lds_rows, p->lds_pairs);
+        const bool lds = p->variant == GCT_BP_LDS && shape >= 0;
+        info->view_block = lds ? kLdsShapes[shape].vb : 0u;
+        info->lds_rows = lds ? kLdsShapes[shape].rows : 0u;
+        info->lds_pairs = lds ? kLdsShapes[shape].pairs : 0u;
     });
 }
 
@@ -724,6 +737,8 @@ extern "C" gct_status gct_reconstruct_spiral(const gct_ct_geom *geom, const floa
         o.bp_kernel = GCT_BP_LDS;
         o.z_unroll = 0;
         o.view_block = 0;
+        o.lds_rows = 0;
+        o.lds_pairs = 0;
         DeviceGuard dg(o.device);
         auto plan = make_plan(*geom, *img, o, geom->num_proj, nullptr, false, &win);
         hipStream_t s = nullptr;
@@ -751,16 +766,19 @@ extern "C" gct_status gct_plan_autotune(gct_plan *p, float *vol, uint32_t *best_
         hipEvent_t e0, e1;
         hip_check(hipEventCreate(&e0), "event");
         hip_check(hipEventCreate(&e1), "event");
-        struct Cand { int variant; uint32_t zu, vb; };
-        const Cand cands[] = {{GCT_BP_LDS, 16u, 2u}, {GCT_BP_LDS, 16u, 3u}, {GCT_BP_LDS, 16u, 4u},
-                              {GCT_BP_LDS, 32u, 1u}, {GCT_BP_LDS, 32u, 2u}, {GCT_BP_DIRECT, 8u, 0u},
-                              {GCT_BP_DIRECT, 16u, 0u}};
+        struct Cand { int variant; uint32_t zu, vb, rows, pairs; };
+        std::vector<Cand> cands;
+        for (const LdsShape &sh : kLdsShapes) cands.push_back({GCT_BP_LDS, sh.zu, sh.vb, sh.rows, sh.pairs});
+        cands.push_back({GCT_BP_DIRECT, 8u, 0u, 0u, 0u});
+        cands.push_back({GCT_BP_DIRECT, 16u, 0u, 0u, 0u});
         float best = 1e30f;
-        Cand bc{p->variant, p->z_unroll, p->view_block};
+        Cand bc{p->variant, p->z_unroll, p->view_block, p->lds_rows, p->lds_pairs};
         for (const Cand &c : cands) {
             p->z_unroll = c.zu;
             p->variant = c.variant;
             p->view_block = c.vb;
+            p->lds_rows = c.rows;
+            p->lds_pairs = c.pairs;
             const BpLaunch a = bp_args(*p, vol, 0, p->V, false, true, 0);
             launch_backproject(a, s);   // warm-up
             float t[5];
@@ -777,6 +795,8 @@ extern "C" gct_status gct_plan_autotune(gct_plan *p, float *vol, uint32_t *best_
         p->variant = bc.variant;
         p->z_unroll = bc.zu;
         p->view_block = bc.vb;
+        p->lds_rows = bc.rows;
+        p->lds_pairs = bc.pairs;
         const uint32_t bz = bc.zu;
         const int bv = bc.variant;
         (void)hipEventDestroy(e0);
@@ -785,7 +805,7 @@ extern "C" gct_status gct_plan_autotune(gct_plan *p, float *vol, uint32_t *best_
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             g_tune[{p->geom.kind, p->img.x.num_samples, p->img.y.num_samples, p->img.dim == 3 ? p->img.z.num_samples : 1u,
-                    p->V, p->nu, p->nv, p->opts.storage, int(p->circular), int(p->setup_f64)}] = Tuning{bz, bv, bc.vb};
+                    p->V, p->nu, p->nv, p->opts.storage, int(p->circular), int(p->setup_f64)}] = Tuning{bz, bv, bc.vb, bc.rows, bc.pairs};
         }
         if (best_zu) *best_zu = bz;
         if (best_ms) *best_ms = best;

@@ -86,7 +86,8 @@ struct gct_plan {
     float inv_dgamma = 0.0f;
     float scale = 1.0f;
     uint32_t z_unroll = 16;
-    uint32_t view_block = 0;   // LDS kernel views per staged block; 0: default for z_unroll
+    uint32_t view_block = 0;   // LDS kernel shape as requested (0: default of z_unroll; see kLdsShapes)
+    uint32_t lds_rows = 0, lds_pairs = 0;
     int variant = GCT_BP_LDS;
     hipStream_t stream = nullptr;     // stream-ordered plans (serving) allocate and free on it
     bool ordered = false;

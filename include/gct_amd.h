@@ -140,7 +140,12 @@ typedef struct gct_recon_opts {
     uint32_t z_unroll;               /* BP voxels per thread along z; 0 = tuned */
     uint32_t view_chunk;             /* LOW mode views per chunk; 0 = derived */
     int32_t bp_kernel;               /* gct_bp_kernel, default AUTO */
-    uint32_t view_block;             /* LDS BP views staged per barrier: 2..4 (z_unroll 16), 1..2 (32); 0 = tuned */
+    /* LDS backprojection shape (z_unroll, view_block, lds_rows, lds_pairs): one of
+     * (16, 2, 40, 40), (16, 2, 32, 40), (16, 2, 40, 48), (32, 1, 64, 40), (32, 2, 64, 48);
+     * a zero field matches any value, all zero = the first shape of z_unroll (or the tuned one) */
+    uint32_t view_block;             /* views staged per block */
+    uint32_t lds_rows;               /* detector rows staged per view */
+    uint32_t lds_pairs;              /* (value, difference) pairs staged per row */
 } gct_recon_opts;
 
 /* ---- library ----------------------------------------------------------- */
@@ -233,7 +238,9 @@ typedef struct gct_plan_info {
     int32_t circular;                /* 1 when every view keeps the row axis along z */
     int32_t storage;
     int32_t bp_kernel;               /* gct_bp_kernel in use */
-    uint32_t view_block;             /* LDS BP views per staged block in use */
+    uint32_t view_block;             /* LDS BP shape in use (0 for the direct kernel): views per block, */
+    uint32_t lds_rows;               /*   staged rows per view, */
+    uint32_t lds_pairs;              /*   pairs per staged row */
 } gct_plan_info;
 
 /* Builds per-view tables (FP64 on the host, stored FP32), the redundancy

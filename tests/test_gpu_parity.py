@@ -34,6 +34,8 @@ def _report(name, got, want):
 
 
 KINDS = [g.PARALLEL_1D, g.FAN_ARC_1D, g.FAN_LINE_1D, g.PARALLEL_2D, g.CONE_PARALLEL, g.CYLINDRICAL, g.FLAT]
+# the LDS backprojection shapes (z unroll, views per block, staged rows, pairs per row)
+LDS_SHAPES = [(16, 2, 40, 40), (16, 2, 32, 40), (16, 2, 40, 48), (32, 1, 64, 40), (32, 2, 64, 48)]
 
 
 @pytest.mark.parametrize("kind", KINDS, ids=g.KIND_NAMES)
@@ -152,19 +154,22 @@ def test_one_shot_high_low_memory_identical(torch_dev, mode):
 @pytest.mark.parametrize("kind", KINDS, ids=g.KIND_NAMES)
 def test_lds_and_direct_kernels_identical(torch_dev, kind, setup):
     """The LDS-staged and the direct backprojector read the same taps and run
-    the same arithmetic per update, and every voxel's row coordinate is stepped
-    from z index 0, so they agree bit for bit whatever the z unroll and view
-    block, with FP32 and with FP64 per-column set-up."""
+    the same arithmetic per update, and every voxel's row coordinate is
+    evaluated from its 16-slice z block, so they agree bit for bit whatever the
+    z unroll and LDS shape, with FP32 and with FP64 per-column set-up."""
     torch = torch_dev
     wl = W.small_case(kind)
     proj = torch.from_numpy(wl.projections()).cuda()
     out = []
-    for bk, zu, vb in ((g.BP_DIRECT, 8, 0), (g.BP_DIRECT, 4, 0), (g.BP_LDS, 16, 2), (g.BP_LDS, 16, 4),
-                       (g.BP_LDS, 32, 1), (g.BP_LDS, 32, 2)):
-        if g.is_1d(kind) and zu == 32:
+    for bk, zu, shape in [(g.BP_DIRECT, 8, None), (g.BP_DIRECT, 4, None)] + [(g.BP_LDS, sh[0], sh[1:]) for sh in LDS_SHAPES]:
+        if g.is_1d(kind) and (zu == 32 or shape not in (None, LDS_SHAPES[0][1:])):
             continue
-        plan = g.Plan(wl.scan, wl.img, bp_kernel=bk, z_unroll=0 if g.is_1d(kind) else zu,
-                      view_block=vb if bk == g.BP_LDS and not g.is_1d(kind) else 0, setup_f64=setup)
+        vb, rows, pairs = shape if shape and not g.is_1d(kind) else (0, 0, 0)
+        plan = g.Plan(wl.scan, wl.img, bp_kernel=bk, z_unroll=0 if g.is_1d(kind) else zu, view_block=vb,
+                      lds_rows=rows, lds_pairs=pairs, setup_f64=setup)
+        if bk == g.BP_LDS and not g.is_1d(kind):
+            i = plan.info()
+            assert (i.z_unroll, i.view_block, i.lds_rows, i.lds_pairs) == (zu, vb, rows, pairs)
         v = torch.empty(wl.img.shape, dtype=torch.float32, device="cuda")
         plan.execute(proj, v)
         torch.cuda.synchronize()
@@ -182,8 +187,9 @@ def test_lds_and_direct_kernels_identical_512(torch_dev):
     wl = W.carm_short_scan_512()
     proj = wl.projections_device()
     ref = None
-    for bk, zu, vb in ((g.BP_DIRECT, 8, 0), (g.BP_LDS, 16, 2), (g.BP_LDS, 32, 2)):
-        plan = g.Plan(wl.scan, wl.img, window=wl.window, bp_kernel=bk, z_unroll=zu, view_block=vb)
+    for bk, zu, (vb, rows, pairs) in [(g.BP_DIRECT, 8, (0, 0, 0))] + [(g.BP_LDS, sh[0], sh[1:]) for sh in LDS_SHAPES]:
+        plan = g.Plan(wl.scan, wl.img, window=wl.window, bp_kernel=bk, z_unroll=zu, view_block=vb, lds_rows=rows,
+                      lds_pairs=pairs)
         assert plan.info().setup_f64 == 1
         v = torch.empty(wl.img.shape, dtype=torch.float32, device="cuda")
         plan.execute(proj, v)
@@ -193,3 +199,21 @@ def test_lds_and_direct_kernels_identical_512(torch_dev):
             ref = v
         else:
             assert torch.equal(v, ref), f"kernel {bk} z_unroll {zu}: {(v != ref).sum().item()} voxels differ"
+
+
+def test_lds_shape_options(torch_dev):
+    """Shape fields left at zero take the first matching shape; a combination
+    that is no shape is rejected with an invalid-argument error."""
+    wl = W.small_case(g.FLAT)
+    cases = [(dict(), (16, 2, 40, 40)), (dict(z_unroll=32), (32, 1, 64, 40)),
+             (dict(lds_rows=32), (16, 2, 32, 40)), (dict(lds_pairs=48), (16, 2, 40, 48)),
+             (dict(z_unroll=32, view_block=2), (32, 2, 64, 48))]
+    for kw, want in cases:
+        plan = g.Plan(wl.scan, wl.img, bp_kernel=g.BP_LDS, **kw)
+        i = plan.info()
+        assert (i.z_unroll, i.view_block, i.lds_rows, i.lds_pairs) == want, kw
+        plan.close()
+    for kw in (dict(view_block=3), dict(z_unroll=32, lds_rows=40), dict(lds_pairs=32)):
+        with pytest.raises(g.GctError) as e:
+            g.Plan(wl.scan, wl.img, bp_kernel=g.BP_LDS, **kw)
+        assert e.value.status == 1 and "no LDS backprojection shape" in str(e.value), kw
