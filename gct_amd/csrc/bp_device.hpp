@@ -528,7 +528,7 @@ __device__ inline void stage_block(const typename Stor<S>::T *base, size_t view_
 // One 16 x 16 x ZU tile (bx, by, bz) of a circular 3-D scan; every thread of
 // the 256-thread workgroup calls it (it holds workgroup barriers).  `tile`
 // and `boxes` are the workgroup's LDS staging area of VB * TR * TP pairs
-// (TR rows of TP pairs per view) and VB boxes.
+// (TR rows of TP pairs per view) and two slots of VB boxes.
 // WIN: spiral scans -- slice z only takes the views [zwin[2z], zwin[2z+1]);
 // the tile's view loop covers the union of its slices' windows, and the
 // per-slice test is on wave-uniform values (scalar branches).
@@ -575,17 +575,23 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
 
     const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(tid >> 6));
     const uint32_t lane = tid & 63u;
-    for (uint32_t vb0 = vfirst; vb0 < vend; vb0 += VB) {
+    // Two barriers per view block: the boxes of block b + 1 are found after
+    // block b is backprojected, into the other of the two box slots, so they
+    // are ready at the barrier that also frees the staging area.
+    if (vfirst < vend) tile_boxes<KIND, ZU>(a, bx, by, bz, vfirst, min(static_cast<uint32_t>(VB), vend - vfirst),
+                                            TR, TP, boxes, tid);
+    __syncthreads();
+    uint32_t cur = 0;
+    for (uint32_t vb0 = vfirst; vb0 < vend; vb0 += VB, cur ^= 1u) {
         const uint32_t nvb = min(static_cast<uint32_t>(VB), vend - vb0);
-        tile_boxes<KIND, ZU>(a, bx, by, bz, vb0, nvb, TR, TP, boxes, tid);
-        __syncthreads();
-        stage_block<S, TR, VB, TP>(base, view_elems, a.pitch, vb0, nvb, boxes, tile, wave, lane);
+        const Box *bc = boxes + cur * VB;
+        stage_block<S, TR, VB, TP>(base, view_elems, a.pitch, vb0, nvb, bc, tile, wave, lane);
         __syncthreads();
         if (active) {
             for (uint32_t j = 0; j < nvb; ++j) {
                 const uint32_t v = vb0 + j;
                 const ColSetup s = setup_column<KIND, F64, false>(a, v, fx, fy, 0.0f, nuf);
-                const Box b = boxes[j];
+                const Box b = bc[j];
                 if (b.ncols != 0) {
                     // Padded coordinates, as in col_sample / bilinear.  The box clamps never
                     // bind for a coordinate of this tile, which the box covers with a margin.
@@ -655,6 +661,10 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                 }
             }
         }
+        const uint32_t vn = vb0 + VB;
+        if (vn < vend)
+            tile_boxes<KIND, ZU>(a, bx, by, bz, vn, min(static_cast<uint32_t>(VB), vend - vn), TR, TP,
+                                 boxes + (cur ^ 1u) * VB, tid);
         __syncthreads();
     }
     if (!active) return;
@@ -673,7 +683,7 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
 template <int ZU, int TR, int VB, int TP>
 constexpr int lds_waves_per_simd()
 {
-    constexpr int bytes = VB * TR * TP * 8 + VB * static_cast<int>(sizeof(Box));
+    constexpr int bytes = VB * TR * TP * 8 + 2 * VB * static_cast<int>(sizeof(Box));
     constexpr int by_lds = (160 * 1024) / bytes;
     constexpr int by_regs = ZU >= 32 ? 3 : 7;
     return by_lds < 1 ? 1 : (by_lds < by_regs ? by_lds : by_regs);
@@ -684,7 +694,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lds_waves_p
 void bp3d_lds_kernel(BpLaunch a)
 {
     __shared__ float2 tile[VB * TR * TP];
-    __shared__ Box boxes[VB];
+    __shared__ Box boxes[2 * VB];
     const uint32_t nbx = (a.nx + 15u) / 16u, nby = (a.ny + 15u) / 16u;
     const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t bx = t % nbx, rest = t / nbx;
@@ -770,7 +780,7 @@ void bp_batch_kernel(const BatchScan *__restrict__ scans, uint32_t nscans,
     constexpr bool k1d = KIND <= GCT_FAN_LINE_1D;
     __shared__ uint32_t item;
     __shared__ float2 tile[LDS ? kBatchVB * kBatchTR * kBatchTP : 1];
-    __shared__ Box boxes[kBatchVB];
+    __shared__ Box boxes[2 * kBatchVB];
     const uint32_t tid = threadIdx.x;
     for (;;) {
         if (tid == 0) item = atomicAdd(counter, 1u);
