@@ -182,9 +182,25 @@ struct FpLaunch {
     float du, dv;
     const gct_proj_geom *views;   // device copy
     const float *vol;
-    float *vol_t;                 // scratch for the x/y-transposed volume ([z][x][y]); null: read `vol` only
+    float *pad;                   // scratch for the zero-padded copies (joseph_pad_elems); null: read `vol` only
     float *proj;
 };
+
+// The forward projector's scratch: two zero-padded copies of the volume, one
+// per dominant ray axis, so that the plane march clamps its taps instead of
+// bounds-checking them.  Rays marching along y read [z+3][y][x+3] (voxel
+// (x, y, z) at ((z+1) ny + y)(nx+3) + x+1); rays along x read the transposed
+// [z+3][x][y+3] copy that follows it.  Returns 0 when the copies do not
+// apply (2-D images) or do not fit 32-bit byte offsets and 24-bit strides;
+// the projector then reads the volume with bounds checks.
+inline uint64_t joseph_pad_elems(int kind, uint32_t nx, uint32_t ny, uint32_t nz)
+{
+    if (kind <= GCT_FAN_LINE_1D) return 0;
+    const uint64_t sy = uint64_t(ny) * (nx + 3u), sx = uint64_t(nx) * (ny + 3u);
+    const uint64_t n = (uint64_t(nz) + 3u) * (sy + sx);
+    if (n >= (uint64_t(1) << 30) || sy >= (uint64_t(1) << 24) || sx >= (uint64_t(1) << 24)) return 0;
+    return n;
+}
 
 void launch_redundancy(int kind, const float *fan_ang, const float *beta, uint32_t nu, uint32_t nviews,
                        float range, int parallel_value_mode, float parallel_value, float *red, hipStream_t s);

@@ -387,17 +387,53 @@ __device__ float joseph_ray(const FpLaunch &a, const float o[3], const float d[3
         else i1 = min(i1, static_cast<int>(ceil(ic)) - 1);
     }
     if (i1 < i0) return 0.0f;
-    const float fa0 = static_cast<float>((o[oa] + t0 * d[oa] - first[oa]) / sp[oa]);
-    const float dfa = static_cast<float>(dt * d[oa] / sp[oa]);
     const float len = sqrtf(d[0] * d[0] + d[1] * d[1] + (is3 ? d[2] * d[2] : 0.0f));
     const float step = sp[ax] * len / fabsf(d[ax]);
-    // Rays marching along x read the x/y-transposed copy, so that the lanes of
-    // a wave (neighbouring detector columns) always spread along the
-    // contiguous axis of the copy they read.
-    const bool tr = ax == 0 && a.vol_t != nullptr;
-    const uint32_t stride[3] = {tr ? a.ny : 1u, tr ? 1u : a.nx, a.nx * a.ny};
+    if (is3 && ax != 2 && a.pad != nullptr) {
+        // Rays marching along x or y read the zero-padded copy for their axis
+        // (joseph_pad_elems): in-plane a = the other of x, y (contiguous in the
+        // copy), b = z.  Padded coordinates (+1) clamped to [0, n + 1] keep both
+        // taps inside the copy and give exactly the bounds-checked taps: a tap
+        // outside the volume lands on zero padding.  Offsets are 32-bit from the
+        // copies' base, the z stride fits 24 bits (checked on the host).
+        const bool xm = ax == 0;
+        const uint32_t na = xm ? a.ny : a.nx, w = na + 3u;
+        const uint32_t sb = (xm ? a.nx : a.ny) * w;
+        const uint32_t base = xm ? (a.nz + 3u) * (a.ny * (a.nx + 3u)) : 0u;
+        const float fa0p = static_cast<float>((o[oa] + t0 * d[oa] - first[oa]) / sp[oa] + 1.0);
+        const float dfa = static_cast<float>(dt * d[oa] / sp[oa]);
+        const float fb0p = static_cast<float>((o[2] + t0 * d[2] - first[2]) / sp[2] + 1.0);
+        const float dfb = static_cast<float>(dt * d[2] / sp[2]);
+        const float nap = static_cast<float>(na + 1u), nbp = static_cast<float>(a.nz + 1u);
+        // byte offsets (the copies hold < 2^30 floats): each tap is one load from
+        // the uniform base plus a 32-bit offset, the +1 column an immediate
+        const char *__restrict__ P = reinterpret_cast<const char *>(a.pad);
+        const uint32_t sb4 = sb * 4u, w4 = w * 4u;
+        uint32_t pb = (base + static_cast<uint32_t>(i0) * w) * 4u;
+        float acc = 0.0f;
+#pragma unroll 8   // independent planes: eight in flight per thread
+        for (int i = i0; i <= i1; ++i, pb += w4) {
+            const float fi = static_cast<float>(i);
+            const float ca = __builtin_amdgcn_fmed3f(fmaf(fi, dfa, fa0p), 0.0f, nap);
+            const float cb = __builtin_amdgcn_fmed3f(fmaf(fi, dfb, fb0p), 0.0f, nbp);
+            const uint32_t b00 = (__umul24(static_cast<uint32_t>(cb), sb) + static_cast<uint32_t>(ca)) * 4u + pb;
+            const float wa = __builtin_amdgcn_fractf(ca), wb = __builtin_amdgcn_fractf(cb);
+            const uint32_t b01 = b00 + sb4;
+            const float v00 = *reinterpret_cast<const float *>(P + b00);
+            const float v10 = *reinterpret_cast<const float *>(P + b00 + 4);
+            const float v01 = *reinterpret_cast<const float *>(P + b01);
+            const float v11 = *reinterpret_cast<const float *>(P + b01 + 4);
+            const float lo = fmaf(wa, v10 - v00, v00), hi = fmaf(wa, v11 - v01, v01);
+            acc += fmaf(wb, hi - lo, lo);
+        }
+        return acc * step;
+    }
+    const float fa0 = static_cast<float>((o[oa] + t0 * d[oa] - first[oa]) / sp[oa]);
+    const float dfa = static_cast<float>(dt * d[oa] / sp[oa]);
+    // the volume itself, with bounds-checked taps
+    const uint32_t stride[3] = {1u, a.nx, a.nx * a.ny};
     const uint32_t na = static_cast<uint32_t>(n[oa]), sa = stride[oa];
-    const float *plane = (tr ? a.vol_t : a.vol) + static_cast<size_t>(i0) * stride[ax];
+    const float *plane = a.vol + static_cast<size_t>(i0) * stride[ax];
     const size_t pstep = stride[ax];
     float acc = 0.0f;
     if (!is3) {
@@ -436,26 +472,44 @@ __device__ float joseph_ray(const FpLaunch &a, const float o[3], const float d[3
     return acc * step;
 }
 
-// [z][y][x] -> [z][x][y], 32 x 32 tiles through LDS (both sides coalesced).
-__global__ __launch_bounds__(256) void transpose_xy_kernel(const float *__restrict__ src, float *__restrict__ dst,
-                                                           uint32_t nx, uint32_t ny, uint32_t tiles_x, uint32_t tiles_y)
+// The forward projector's zero-padded copies (joseph_pad_elems).
+// y-march copy [z+3][y][x+3]: one padded row per workgroup, x fastest on both sides.
+__global__ __launch_bounds__(256) void pad_ymarch_kernel(const float *__restrict__ src, float *__restrict__ dst,
+                                                         uint32_t nx, uint32_t ny, uint32_t nz)
+{
+    const uint32_t row = blockIdx.x, w = nx + 3u;
+    const uint32_t zp = row / ny, y = row - zp * ny;
+    const bool zin = zp >= 1u && zp <= nz;
+    const float *sr = src + (static_cast<size_t>(zin ? zp - 1u : 0u) * ny + y) * nx;
+    float *dr = dst + static_cast<size_t>(row) * w;
+    for (uint32_t xp = threadIdx.x; xp < w; xp += 256u)
+        dr[xp] = (zin && xp >= 1u && xp <= nx) ? sr[xp - 1u] : 0.0f;
+}
+
+// x-march copy [z+3][x][y+3], written after the y-march copy: 32 x 32 tiles
+// of (x, padded y) through LDS, so both the reads (x fastest) and the writes
+// (y fastest) are coalesced.
+__global__ __launch_bounds__(256) void pad_xmarch_kernel(const float *__restrict__ src, float *__restrict__ dst,
+                                                         uint32_t nx, uint32_t ny, uint32_t nz, uint32_t tiles_x,
+                                                         uint32_t tiles_y)
 {
     __shared__ float t[32][33];
     const uint32_t b = blockIdx.x;
     const uint32_t tx = b % tiles_x, rest = b / tiles_x;
-    const uint32_t ty = rest % tiles_y, z = rest / tiles_y;
-    const size_t slice = static_cast<size_t>(nx) * ny;
-    const float *sp = src + z * slice;
-    float *dp = dst + z * slice;
+    const uint32_t ty = rest % tiles_y, zp = rest / tiles_y;
+    const uint32_t w = ny + 3u;
+    const bool zin = zp >= 1u && zp <= nz;
+    const float *sp = src + static_cast<size_t>(zin ? zp - 1u : 0u) * nx * ny;
+    float *dp = dst + static_cast<size_t>(zp) * nx * w;
     const uint32_t lx = threadIdx.x & 31u, ly = threadIdx.x >> 5;
     for (uint32_t k = 0; k < 32u; k += 8u) {
-        const uint32_t x = tx * 32u + lx, y = ty * 32u + ly + k;
-        if (x < nx && y < ny) t[ly + k][lx] = sp[static_cast<size_t>(y) * nx + x];
+        const uint32_t x = tx * 32u + lx, yp = ty * 32u + ly + k;
+        t[ly + k][lx] = (zin && x < nx && yp >= 1u && yp <= ny) ? sp[static_cast<size_t>(yp - 1u) * nx + x] : 0.0f;
     }
     __syncthreads();
     for (uint32_t k = 0; k < 32u; k += 8u) {
-        const uint32_t y = ty * 32u + lx, x = tx * 32u + ly + k;
-        if (x < nx && y < ny) dp[static_cast<size_t>(x) * ny + y] = t[lx][ly + k];
+        const uint32_t yp = ty * 32u + lx, x = tx * 32u + ly + k;
+        if (x < nx && yp < w) dp[static_cast<size_t>(x) * w + yp] = t[lx][ly + k];
     }
 }
 
@@ -639,10 +693,11 @@ void launch_forward_project(const FpLaunch &a, hipStream_t s)
 {
     const size_t total = static_cast<size_t>(a.nviews) * (is_1d(a.kind) ? 1u : a.nv) * a.nu;
     if (total == 0) return;
-    if (a.vol_t) {
-        const uint32_t tx = (a.nx + 31u) / 32u, ty = (a.ny + 31u) / 32u;
-        hipLaunchKernelGGL(transpose_xy_kernel, dim3(tx * ty * a.nz), dim3(256), 0, s, a.vol, a.vol_t, a.nx, a.ny, tx,
-                           ty);
+    if (a.pad) {
+        hipLaunchKernelGGL(pad_ymarch_kernel, dim3((a.nz + 3u) * a.ny), dim3(256), 0, s, a.vol, a.pad, a.nx, a.ny, a.nz);
+        const uint32_t tx = (a.nx + 31u) / 32u, ty = (a.ny + 3u + 31u) / 32u;
+        hipLaunchKernelGGL(pad_xmarch_kernel, dim3(tx * ty * (a.nz + 3u)), dim3(256), 0, s, a.vol,
+                           a.pad + static_cast<size_t>(a.nz + 3u) * a.ny * (a.nx + 3u), a.nx, a.ny, a.nz, tx, ty);
     }
     hipLaunchKernelGGL(joseph_kernel, dim3(static_cast<uint32_t>((total + 255) / 256)), dim3(256), 0, s, a);
     hip_check(hipGetLastError(), "forward projection");

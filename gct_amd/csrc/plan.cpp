@@ -418,7 +418,7 @@ void check_range(const gct_plan &p, uint32_t first, uint32_t n)
 
 }  // namespace
 
-FpLaunch fp_args(gct_plan &p, const float *vol, float *proj, float *vol_t)
+FpLaunch fp_args(gct_plan &p, const float *vol, float *proj, float *pad)
 {
     FpLaunch a{};
     a.kind = p.geom.kind;
@@ -440,7 +440,7 @@ FpLaunch fp_args(gct_plan &p, const float *vol, float *proj, float *vol_t)
     a.dv = is_1d(p.geom.kind) ? 1.0f : p.geom.det_col.sampling_width;
     a.views = p.d_views.as<gct_proj_geom>();
     a.vol = vol;
-    a.vol_t = vol_t;
+    a.pad = joseph_pad_elems(a.kind, a.nx, a.ny, a.nz) ? pad : nullptr;
     a.proj = proj;
     return a;
 }
@@ -619,9 +619,11 @@ extern "C" gct_status gct_plan_forward_project(gct_plan *p, const float *vol, fl
     return guarded([&] {
         if (!p || !vol || !proj) fail(GCT_ERR_INVALID_ARGUMENT, "null argument");
         DeviceGuard dg(p->device);
-        // the transposed-volume scratch is allocated on first use and kept with the plan
-        if (!p->d_vol_t.p) p->d_vol_t = DevBuf(volume_bytes(p->img));
-        const FpLaunch a = fp_args(*p, vol, proj, p->d_vol_t.as<float>());
+        // the padded-copy scratch is allocated on first use and kept with the plan
+        const uint64_t pad = joseph_pad_elems(p->geom.kind, p->img.x.num_samples, p->img.y.num_samples,
+                                              p->img.dim == 3 ? p->img.z.num_samples : 1u);
+        if (pad && !p->d_vol_pad.p) p->d_vol_pad = DevBuf(pad * sizeof(float));
+        const FpLaunch a = fp_args(*p, vol, proj, p->d_vol_pad.as<float>());
         launch_forward_project(a, static_cast<hipStream_t>(stream));
     });
 }

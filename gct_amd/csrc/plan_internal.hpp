@@ -92,7 +92,7 @@ struct gct_plan {
     hipStream_t stream = nullptr;     // stream-ordered plans (serving) allocate and free on it
     bool ordered = false;
     DevBuf d_brec, d_brec64, d_frec, d_red, d_H, d_tw, d_filt, d_views;
-    DevBuf d_vol_t;   // forward projection: x/y-transposed volume, allocated on first use
+    DevBuf d_vol_pad;   // forward projection: zero-padded volume copies, allocated on first use
     bool spiral = false;   // spiral FDK: no redundancy weighting, per-slice view windows
     DevBuf d_zwin;         // spiral: [first, last) per z slice
 };
@@ -107,7 +107,7 @@ FilterLaunch filter_args(gct_plan &p, const float *proj, uint32_t proj_view_offs
                          uint32_t filt_view_offset);
 BpLaunch bp_args(gct_plan &p, float *vol, uint32_t first, uint32_t n, bool accumulate, bool finish,
                  uint32_t filt_view_offset);
-FpLaunch fp_args(gct_plan &p, const float *vol, float *proj, float *vol_t);
+FpLaunch fp_args(gct_plan &p, const float *vol, float *proj, float *pad);
 uint64_t plan_device_bytes(const gct_ct_geom &g, const gct_image_geom &img, const gct_recon_opts &o, uint32_t fviews);
 uint64_t volume_bytes(const gct_image_geom &img);
 size_t storage_bytes(int storage);

@@ -57,7 +57,10 @@ uint64_t request_footprint(const gct_ct_geom &g, const gct_image_geom &img, cons
     const uint64_t nv = is_1d(g.kind) ? 1u : g.det_col.num_samples;
     const uint64_t proj = uint64_t(g.num_proj) * nv * g.det_row.num_samples * sizeof(float);
     uint64_t b = proj + volume_bytes(img) + plan_device_bytes(g, img, o, 0);
-    if (reproject) b += proj + volume_bytes(img) + 2u * kResidualBlocks * sizeof(double);
+    if (reproject)
+        b += proj + joseph_pad_elems(g.kind, img.x.num_samples, img.y.num_samples, img.dim == 3 ? img.z.num_samples : 1u) *
+                        sizeof(float) +
+             2u * kResidualBlocks * sizeof(double);
     return b;
 }
 
@@ -104,7 +107,9 @@ void gct_server::process(Request &r, hipStream_t s)
         DevBuf dre, dvt, dpart;
         if (r.reproject) {
             dre = DevBuf(npix * sizeof(float), s, true);
-            dvt = DevBuf(volume_bytes(r.img), s, true);
+            const uint64_t pad = joseph_pad_elems(r.geom.kind, r.img.x.num_samples, r.img.y.num_samples,
+                                                  r.img.dim == 3 ? r.img.z.num_samples : 1u);
+            if (pad) dvt = DevBuf(pad * sizeof(float), s, true);
             dpart = DevBuf(2u * kResidualBlocks * sizeof(double), s, true);
             launch_forward_project(fp_args(*plan, dvol.as<float>(), dre.as<float>(), dvt.as<float>()), s);
             launch_residual(dre.as<float>(), dproj.as<float>(), npix, dpart.as<double>(), kResidualBlocks, s);
