@@ -139,8 +139,17 @@ template <> __device__ inline void dft<16>(float2 *u)
     }
 }
 
-template <uint32_t L, uint32_t R, uint32_t P, uint32_t NT>
-__device__ inline void stockham_pass(float2 *buf, const float2 *tw, uint32_t tid)
+// One radix-R Stockham pass over L points, P the product of the radices
+// already done.  Butterfly i (0 <= i < L/R) takes points i + j L/R and gives
+// points (i - k) R + k + j P, k = i mod P.  `ld(i, j)` supplies point
+// i + j L/R and `st(idx, j, v)` takes output point idx: the group's LDS
+// buffer for inner passes; global memory, the weights and the ramp spectrum
+// at the ends, which saves the LDS round trips and barriers of separate load,
+// multiply and store stages.  SYNC_MID: the pass reads and writes the same
+// buffer, so every read completes before the first write; SYNC_END: the next
+// pass reads what this one wrote.
+template <uint32_t L, uint32_t R, uint32_t P, uint32_t NT, bool SYNC_MID, bool SYNC_END, class Ld, class St>
+__device__ inline void stockham_io(const float2 *tw, uint32_t tid, Ld ld, St st)
 {
     constexpr uint32_t T = L / R;
     constexpr uint32_t B = (T + NT - 1) / NT;
@@ -150,10 +159,10 @@ __device__ inline void stockham_pass(float2 *buf, const float2 *tw, uint32_t tid
         const uint32_t i = tid + b * NT;
         if (T % NT == 0 || i < T) {
 #pragma unroll
-            for (uint32_t j = 0; j < R; ++j) u[b][j] = buf[i + j * T];
+            for (uint32_t j = 0; j < R; ++j) u[b][j] = ld(i, j);
         }
     }
-    __syncthreads();
+    if constexpr (SYNC_MID) __syncthreads();
 #pragma unroll
     for (uint32_t b = 0; b < B; ++b) {
         const uint32_t i = tid + b * NT;
@@ -167,10 +176,10 @@ __device__ inline void stockham_pass(float2 *buf, const float2 *tw, uint32_t tid
             dft<R>(u[b]);
             const uint32_t base = (i - k) * R + k;
 #pragma unroll
-            for (uint32_t j = 0; j < R; ++j) buf[base + j * P] = u[b][j];
+            for (uint32_t j = 0; j < R; ++j) st(base + j * P, j, u[b][j]);
         }
     }
-    __syncthreads();
+    if constexpr (SYNC_END) __syncthreads();
 }
 
 // Radix order: 16 while at least 16 remain (8 then 4 for 32), then the rest.
@@ -180,13 +189,42 @@ constexpr uint32_t next_radix()
     return rem == 32u ? 8u : (rem >= 16u ? 16u : rem);
 }
 
-template <uint32_t L, uint32_t P, uint32_t NT>
-__device__ inline void fft_from(float2 *buf, const float2 *tw, uint32_t tid)
+// The radix of the last pass of an L-point FFT.
+template <uint32_t L, uint32_t P = 1>
+constexpr uint32_t last_radix()
+{
+    if constexpr (P * next_radix<L / P>() >= L)
+        return next_radix<L / P>();
+    else
+        return last_radix<L, P * next_radix<L / P>()>();
+}
+
+// The passes of an L-point FFT from product P on, in the group's LDS buffer
+// `buf`.  The first pass reads through `ld` when LD_CUSTOM (else from
+// `buf`); the last always writes through `st`, which writes `buf` when
+// ST_LDS.  Passes that write `buf` end with a barrier for its next reader.
+template <uint32_t L, uint32_t P, uint32_t NT, bool LD_CUSTOM, bool ST_LDS, class Ld, class St>
+__device__ inline void fft_io(float2 *buf, const float2 *tw, uint32_t tid, Ld ld, St st)
 {
     if constexpr (P < L) {
         constexpr uint32_t R = next_radix<L / P>();
-        stockham_pass<L, R, P, NT>(buf, tw, tid);
-        fft_from<L, P * R, NT>(buf, tw, tid);
+        constexpr bool first = P == 1u, last = P * R == L;
+        auto lds_ld = [buf](uint32_t i, uint32_t j) { return buf[i + j * (L / R)]; };
+        auto lds_st = [buf](uint32_t idx, uint32_t, float2 v) { buf[idx] = v; };
+        constexpr bool in_lds = !(first && LD_CUSTOM), out_lds = !last || ST_LDS;
+        constexpr bool sync_mid = in_lds && out_lds, sync_end = out_lds;
+        if constexpr (first && LD_CUSTOM) {
+            if constexpr (last)
+                stockham_io<L, R, P, NT, sync_mid, sync_end>(tw, tid, ld, st);
+            else
+                stockham_io<L, R, P, NT, sync_mid, sync_end>(tw, tid, ld, lds_st);
+        } else {
+            if constexpr (last)
+                stockham_io<L, R, P, NT, sync_mid, sync_end>(tw, tid, lds_ld, st);
+            else
+                stockham_io<L, R, P, NT, sync_mid, sync_end>(tw, tid, lds_ld, lds_st);
+        }
+        fft_io<L, P * R, NT, LD_CUSTOM, ST_LDS>(buf, tw, tid, ld, st);
     }
 }
 
@@ -246,55 +284,47 @@ __global__ __launch_bounds__(kFiltBlock) void filter_kernel(FilterLaunch a)
     const float *red0 = a.red + static_cast<size_t>(v0) * nu;
     const float *red1 = a.red + static_cast<size_t>(v1) * nu;
 
-    // Every global load of a stage is issued before its first use (the loops
-    // are unrolled over the L / NT samples of each thread).
-    constexpr uint32_t KI = L / NT;
-    {
-        float xa[KI], xb[KI], wa[KI], wb[KI];
-#pragma unroll
-        for (uint32_t k = 0; k < KI; ++k) {
-            const uint32_t c = tid + k * NT;
-            const bool in = c < nu && has0;
-            xa[k] = in ? pa[c] : 0.0f;
-            wa[k] = in ? red0[c] : 0.0f;
-            xb[k] = (in && has1) ? pb[c] : 0.0f;
-            wb[k] = (in && has1) ? red1[c] : 0.0f;
-        }
-#pragma unroll
-        for (uint32_t k = 0; k < KI; ++k) {
-            const uint32_t c = tid + k * NT;
-            // fdk_axial.cuh:171-173: p *= cosine weight, then p *= redundancy weight
-            const float x = (xa[k] * cos_weight<KIND>(f0, a.du, a.dv, c, j0)) * wa[k];
-            const float y = (xb[k] * cos_weight<KIND>(f1, a.du, a.dv, c, j1)) * wb[k];
-            buf[c] = make_float2(x, y);
-        }
-    }
-    __syncthreads();
-    fft_from<L, 1, NT>(buf, tw, tid);
-    {
-        float2 h[KI];
-#pragma unroll
-        for (uint32_t k = 0; k < KI; ++k) h[k] = a.H[tid + k * NT];
-#pragma unroll
-        for (uint32_t k = 0; k < KI; ++k) {
-            const uint32_t c = tid + k * NT;
-            const float2 z = cmul(buf[c], h[k]);
-            buf[c] = make_float2(z.x, -z.y);
-        }
-    }
-    __syncthreads();
-    fft_from<L, 1, NT>(buf, tw, tid);
-    if (!has0) return;
+    // Forward FFT: the first pass loads the weighted rows straight from global
+    // memory (points at or beyond L/2 >= nu are zero, so half its inputs are
+    // constant zeros); the last pass multiplies by the ramp spectrum and
+    // conjugates before writing LDS.
+    const float du = a.du, dv = a.dv;
+    auto ld_rows = [&](uint32_t i, uint32_t j) -> float2 {
+        constexpr uint32_t R0 = next_radix<L>();
+        if (j >= R0 / 2u) return make_float2(0.0f, 0.0f);
+        const uint32_t c = i + j * (L / R0);
+        const bool in = c < nu && has0;
+        const float xa = in ? pa[c] : 0.0f, wa = in ? red0[c] : 0.0f;
+        const float xb = (in && has1) ? pb[c] : 0.0f, wb = (in && has1) ? red1[c] : 0.0f;
+        // fdk_axial.cuh:171-173: p *= cosine weight, then p *= redundancy weight
+        return make_float2((xa * cos_weight<KIND>(f0, du, dv, c, j0)) * wa,
+                           (xb * cos_weight<KIND>(f1, du, dv, c, j1)) * wb);
+    };
+    const float2 *__restrict__ H = a.H;
+    auto st_spectrum = [buf, H](uint32_t idx, uint32_t, float2 v) {
+        const float2 z = cmul(v, H[idx]);
+        buf[idx] = make_float2(z.x, -z.y);
+    };
+    // (the first pass uses no twiddles: the barrier that ends it also
+    // publishes the table)
+    fft_io<L, 1, NT, true, true>(buf, tw, tid, ld_rows, st_spectrum);
 
+    // Inverse FFT (conj . FFT . conj): the last pass writes the filtered rows
+    // straight into the padded layout; only points below nu are kept.
     typename St::T *out = static_cast<typename St::T *>(a.filt);
     const size_t view_elems = static_cast<size_t>(a.rows_pv) * a.pitch;
     typename St::T *oa = out + static_cast<size_t>(v0) * view_elems + static_cast<size_t>(j0 + 1u) * a.pitch + 1u;
     typename St::T *ob = out + static_cast<size_t>(v1) * view_elems + static_cast<size_t>(j1 + 1u) * a.pitch + 1u;
-    for (uint32_t c = tid; c < nu; c += NT) {
-        const float2 z = buf[c];
-        oa[c] = St::cv(z.x);
-        if (has1) ob[c] = St::cv(-z.y);
-    }
+    auto st_rows = [&](uint32_t idx, uint32_t j, float2 z) {
+        constexpr uint32_t RL = last_radix<L>();
+        if (j >= RL / 2u) return;   // idx >= L/2 >= nu
+        if (idx < nu && has0) {
+            oa[idx] = St::cv(z.x);
+            if (has1) ob[idx] = St::cv(-z.y);
+        }
+    };
+    auto ld_none = [](uint32_t, uint32_t) { return make_float2(0.0f, 0.0f); };
+    fft_io<L, 1, NT, false, false>(buf, tw, tid, ld_none, st_rows);
 }
 
 __global__ void zero_counters_kernel(uint32_t *c, uint32_t n)
