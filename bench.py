@@ -96,6 +96,17 @@ def cpu_baseline(wl, proj, gpu_vol, nthreads):
 
 # ---- flat-cone plain FDK (the headline) -----------------------------------------
 
+def _bp_kernel_name(g, info):
+    """rocprofv3's name of the backprojection kernel a flat-detector FP32 plan
+    launches (template arguments: kind, storage, FP64 set-up, z unroll, and
+    for the LDS kernel staged rows, views per block, pairs per row)."""
+    f64 = "true" if info.setup_f64 else "false"
+    if info.bp_kernel == g.BP_LDS:
+        return (f"void gct::(anonymous namespace)::bp3d_lds_kernel<6, 0, {f64}, {info.z_unroll}, {info.lds_rows}, "
+                f"{info.view_block}, {info.lds_pairs}, false>(gct::BpLaunch)")
+    return f"void gct::(anonymous namespace)::bp3d_kernel<6, 0, true, {f64}, {info.z_unroll}>(gct::BpLaunch)"
+
+
 def run_flat(a, g, W, torch, np, sections):
     wl = W.flat_cone_256()
     t = time.perf_counter()
@@ -137,12 +148,17 @@ def run_flat(a, g, W, torch, np, sections):
 
     achieved_tflops = F_ALG_FLAT * upd / (bp_mean * 1e-3) / 1e12
     alg_bytes = wl.scan.num_proj * wl.scan.nv * wl.scan.nu * 4 + wl.img.nvox * 4
-    traffic, kernel = None, None
+    # the kernel the tuned plan launches, named as rocprofv3 names it; the
+    # counter-measured traffic applies only when the profile is of that kernel
+    kernel = _bp_kernel_name(g, info)
+    traffic, traffic_kernel = None, None
     tfile = ROOT / "profiles" / "bp_flat_cone_256_traffic.json"
     if tfile.exists():
         try:
             tj = json.loads(tfile.read_text())
-            traffic, kernel = tj.get("hbm_bytes_per_launch"), tj.get("kernel")
+            traffic_kernel = tj.get("kernel")
+            if traffic_kernel == kernel:
+                traffic = tj.get("hbm_bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
 
@@ -181,12 +197,13 @@ def run_flat(a, g, W, torch, np, sections):
         "rmse_vs_phantom": rmse_truth,
         "roofline": {
             "bound": "valu",
-            "kernel": kernel or f"bp3d_{variant}_kernel<FLAT, f32, ZU={info.z_unroll}>",
+            "kernel": kernel,
             "achieved": round(achieved_tflops, 3),
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 4),
             "traffic": traffic,
+            "traffic_profiled_kernel": traffic_kernel,
             "alg_flops_per_update": F_ALG_FLAT,
             "alg_hbm_bytes_per_launch": alg_bytes,
             "hbm_frac_of_peak_alg": round(alg_bytes / (bp_mean * 1e-3) / 1e9 / PEAK_HBM_GBS, 5),

@@ -773,27 +773,37 @@ extern "C" gct_status gct_plan_autotune(gct_plan *p, float *vol, uint32_t *best_
         for (const LdsShape &sh : kLdsShapes) cands.push_back({GCT_BP_LDS, sh.zu, sh.vb, sh.rows, sh.pairs});
         cands.push_back({GCT_BP_DIRECT, 8u, 0u, 0u, 0u});
         cands.push_back({GCT_BP_DIRECT, 16u, 0u, 0u, 0u});
+        // Two interleaved rounds (the second only for candidates within 10 % of
+        // the best so far); a candidate keeps the better of its medians of
+        // five, so drift in clocks or caches does not decide between close ones.
+        std::vector<float> med(cands.size(), 1e30f);
         float best = 1e30f;
-        Cand bc{p->variant, p->z_unroll, p->view_block, p->lds_rows, p->lds_pairs};
-        for (const Cand &c : cands) {
-            p->z_unroll = c.zu;
-            p->variant = c.variant;
-            p->view_block = c.vb;
-            p->lds_rows = c.rows;
-            p->lds_pairs = c.pairs;
-            const BpLaunch a = bp_args(*p, vol, 0, p->V, false, true, 0);
-            launch_backproject(a, s);   // warm-up
-            float t[5];
-            for (float &ms : t) {
-                hip_check(hipEventRecord(e0, s), "record");
-                launch_backproject(a, s);
-                hip_check(hipEventRecord(e1, s), "record");
-                hip_check(hipEventSynchronize(e1), "sync");
-                hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        size_t bi = 0;
+        for (int round = 0; round < 2; ++round) {
+            for (size_t ci = 0; ci < cands.size(); ++ci) {
+                if (round == 1 && med[ci] > 1.1f * best) continue;
+                const Cand &c = cands[ci];
+                p->z_unroll = c.zu;
+                p->variant = c.variant;
+                p->view_block = c.vb;
+                p->lds_rows = c.rows;
+                p->lds_pairs = c.pairs;
+                const BpLaunch a = bp_args(*p, vol, 0, p->V, false, true, 0);
+                launch_backproject(a, s);   // warm-up
+                float t[5];
+                for (float &ms : t) {
+                    hip_check(hipEventRecord(e0, s), "record");
+                    launch_backproject(a, s);
+                    hip_check(hipEventRecord(e1, s), "record");
+                    hip_check(hipEventSynchronize(e1), "sync");
+                    hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+                }
+                std::sort(t, t + 5);
+                med[ci] = std::min(med[ci], t[2]);
+                if (med[ci] < best) { best = med[ci]; bi = ci; }
             }
-            std::sort(t, t + 5);
-            if (t[2] < best) { best = t[2]; bc = c; }   // median of five
         }
+        const Cand bc = cands[bi];
         p->variant = bc.variant;
         p->z_unroll = bc.zu;
         p->view_block = bc.vb;
