@@ -286,9 +286,15 @@ class Image:
 
 
 def opts(**kw) -> ReconOpts:
+    """gct_recon_opts with the library's defaults and the given fields set;
+    an unknown field name is an error (ctypes would otherwise accept it as a
+    plain attribute that never reaches the library)."""
     o = ReconOpts()
     lib().gct_recon_opts_default(C.byref(o))
+    names = {f[0] for f in ReconOpts._fields_}
     for k, v in kw.items():
+        if k not in names:
+            raise TypeError(f"gct_recon_opts has no field {k!r}")
         setattr(o, k, v)
     return o
 

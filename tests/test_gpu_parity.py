@@ -166,7 +166,9 @@ def test_lds_and_direct_kernels_identical(torch_dev, kind, setup):
             continue
         vb, rows, pairs = shape if shape and not g.is_1d(kind) else (0, 0, 0)
         plan = g.Plan(wl.scan, wl.img, bp_kernel=bk, z_unroll=0 if g.is_1d(kind) else zu, view_block=vb,
-                      lds_rows=rows, lds_pairs=pairs, setup_f64=setup)
+                      lds_rows=rows, lds_pairs=pairs, setup_precision=setup)
+        if kind in (g.FLAT, g.CYLINDRICAL):
+            assert plan.info().setup_f64 == (setup == g.SETUP_F64)
         if bk == g.BP_LDS and not g.is_1d(kind):
             i = plan.info()
             assert (i.z_unroll, i.view_block, i.lds_rows, i.lds_pairs) == (zu, vb, rows, pairs)
