@@ -847,12 +847,14 @@ void launch_bp3d_lds(const BpLaunch &a, hipStream_t s)
     static_assert(kLdsShapes[1].zu == 16 && kLdsShapes[1].vb == 2 && kLdsShapes[1].rows == 32 && kLdsShapes[1].pairs == 40);
     static_assert(kLdsShapes[2].zu == 16 && kLdsShapes[2].vb == 2 && kLdsShapes[2].rows == 40 && kLdsShapes[2].pairs == 48);
     static_assert(kLdsShapes[3].zu == 16 && kLdsShapes[3].vb == 3 && kLdsShapes[3].rows == 28 && kLdsShapes[3].pairs == 34);
-    static_assert(kLdsShapes[4].zu == 32 && kLdsShapes[4].vb == 2 && kLdsShapes[4].rows == 64 && kLdsShapes[4].pairs == 48);
+    static_assert(kLdsShapes[4].zu == 16 && kLdsShapes[4].vb == 4 && kLdsShapes[4].rows == 24 && kLdsShapes[4].pairs == 28);
+    static_assert(kLdsShapes[5].zu == 32 && kLdsShapes[5].vb == 2 && kLdsShapes[5].rows == 64 && kLdsShapes[5].pairs == 48);
     switch (shape) {
     case 1: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 32, 2, 40>), g, b, 0, s, a); break;
     case 2: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 2, 48>), g, b, 0, s, a); break;
     case 3: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 28, 3, 34>), g, b, 0, s, a); break;
-    case 4: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 32, 64, 2, 48>), g, b, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 24, 4, 28>), g, b, 0, s, a); break;
+    case 5: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 32, 64, 2, 48>), g, b, 0, s, a); break;
     default: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 2, 40>), g, b, 0, s, a); break;
     }
 }

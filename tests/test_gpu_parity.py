@@ -35,7 +35,7 @@ def _report(name, got, want):
 
 KINDS = [g.PARALLEL_1D, g.FAN_ARC_1D, g.FAN_LINE_1D, g.PARALLEL_2D, g.CONE_PARALLEL, g.CYLINDRICAL, g.FLAT]
 # the LDS backprojection shapes (z unroll, views per block, staged rows, pairs per row)
-LDS_SHAPES = [(16, 2, 40, 40), (16, 2, 32, 40), (16, 2, 40, 48), (16, 3, 28, 34), (32, 2, 64, 48)]
+LDS_SHAPES = [(16, 2, 40, 40), (16, 2, 32, 40), (16, 2, 40, 48), (16, 3, 28, 34), (16, 4, 24, 28), (32, 2, 64, 48)]
 
 
 @pytest.mark.parametrize("kind", KINDS, ids=g.KIND_NAMES)
@@ -215,7 +215,7 @@ def test_lds_shape_options(torch_dev):
         i = plan.info()
         assert (i.z_unroll, i.view_block, i.lds_rows, i.lds_pairs) == want, kw
         plan.close()
-    for kw in (dict(view_block=4), dict(z_unroll=32, lds_rows=40), dict(lds_pairs=32)):
+    for kw in (dict(view_block=5), dict(z_unroll=32, lds_rows=40), dict(lds_pairs=32)):
         with pytest.raises(g.GctError) as e:
             g.Plan(wl.scan, wl.img, bp_kernel=g.BP_LDS, **kw)
         assert e.value.status == 1 and "no LDS backprojection shape" in str(e.value), kw
