@@ -549,10 +549,15 @@ __global__ __launch_bounds__(256) void joseph_kernel(FpLaunch a)
     const uint32_t nv = is_1d(a.kind) ? 1u : a.nv;
     const size_t total = static_cast<size_t>(a.nviews) * nv * a.nu;
     if (idx >= total) return;
+    // Rays are taken detector row by detector row, all views of a row in
+    // turn: the rays of one row meet the volume in one z slab for every view,
+    // so consecutive workgroups reuse that slab from L2 / MALL instead of
+    // sweeping the whole volume once per view.
     const uint32_t c = static_cast<uint32_t>(idx % a.nu);
-    const size_t rr = idx / a.nu;
-    const uint32_t r = static_cast<uint32_t>(rr % nv);
-    const uint32_t v = static_cast<uint32_t>(rr / nv);
+    const size_t rv = idx / a.nu;
+    const uint32_t v = static_cast<uint32_t>(rv % a.nviews);
+    const uint32_t r = static_cast<uint32_t>(rv / a.nviews);
+    const size_t out = (static_cast<size_t>(v) * nv + r) * a.nu + c;
     const gct_proj_geom g = a.views[v];
     const float uc = (static_cast<float>(c) - g.det_row_center) * a.du;
     const float vr = (static_cast<float>(r) - g.det_col_center) * a.dv;
@@ -608,7 +613,7 @@ __global__ __launch_bounds__(256) void joseph_kernel(FpLaunch a)
         break;
     }
     }
-    a.proj[idx] = joseph_ray(a, o, d, from_src);
+    a.proj[out] = joseph_ray(a, o, d, from_src);
 }
 
 // ---- residual of a reprojection -------------------------------------------------
