@@ -217,7 +217,11 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
     if (geom.det_row.num_samples > 2048u) fail(GCT_ERR_UNSUPPORTED, "more than 2048 detector columns");
     if (o.storage < GCT_STORE_F32 || o.storage > GCT_STORE_BF16) fail(GCT_ERR_INVALID_ARGUMENT, "unknown storage");
     if (o.bp_kernel < GCT_BP_AUTO || o.bp_kernel > GCT_BP_LDS) fail(GCT_ERR_INVALID_ARGUMENT, "unknown bp_kernel");
-    const int variant = (o.bp_kernel == GCT_BP_DIRECT) ? GCT_BP_DIRECT : GCT_BP_LDS;
+    // 2-D images default to the direct kernel: the LDS one stages whole
+    // detector rows per view, many times what a 16 x 16 pixel tile reads
+    const int variant = (o.bp_kernel == GCT_BP_DIRECT || (o.bp_kernel == GCT_BP_AUTO && is_1d(geom.kind)))
+                            ? GCT_BP_DIRECT
+                            : GCT_BP_LDS;
     if (o.z_unroll != 0 && !(variant == GCT_BP_DIRECT ? (o.z_unroll == 4 || o.z_unroll == 8 || o.z_unroll == 16)
                                                       : (o.z_unroll == 16 || o.z_unroll == 32)))
         fail(GCT_ERR_INVALID_ARGUMENT, "z_unroll must be 4, 8 or 16 (direct kernel) or 16 or 32 (LDS kernel)");

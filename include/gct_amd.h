@@ -125,7 +125,7 @@ typedef enum gct_mem_mode {
 
 /* Backprojection kernel family. */
 typedef enum gct_bp_kernel {
-    GCT_BP_AUTO = 0,                 /* tuned choice, else LDS-staged */
+    GCT_BP_AUTO = 0,                 /* tuned choice, else LDS-staged (3-D) / direct (2-D images) */
     GCT_BP_DIRECT = 1,               /* taps read from HBM/L2 through the vector cache */
     GCT_BP_LDS = 2                   /* per-tile detector footprint staged in LDS */
 } gct_bp_kernel;
