@@ -177,7 +177,7 @@ __device__ inline ColSetup setup_column(const BpLaunch &a, uint32_t v, float fx,
             s.nv0d = nv;
             s.nv0 = static_cast<float>(nv);
             s.dnv = static_cast<float>(r.v[3]);
-            s.w = static_cast<float>(r.k[3]);
+            s.w = 1.0f;   // the forms are scaled so that the weight is 1 / |r|^2
         } else {
             const BpRec &r = a.rec[v];
             float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
@@ -189,14 +189,14 @@ __device__ inline ColSetup setup_column(const BpLaunch &a, uint32_t v, float fx,
                 nv = fmaf(r.v[3], fz, nv);
             }
             const float rho2 = fmaf(nu, nu, nw * nw);
-            const float rr = 1.0f / sqrtf(rho2);
+            const float rr = __builtin_amdgcn_rsqf(rho2);   // v_rsq_f32 (1 ulp)
             s.col = fmaf(atan2f(nu, nw), a.inv_dgamma, r.k[0]);
             s.row0 = fmaf(r.k[2] * nv, rr, r.k[1]);
             s.drow = r.k[2] * r.v[3] * rr;
             s.rho2 = rho2;
             s.nv0 = nv;
             s.dnv = r.v[3];
-            s.w = r.k[3];   // SID; divided by |r|^2 per voxel
+            s.w = 1.0f;   // the forms are scaled so that the weight is 1 / |r|^2 per voxel
         }
     }
     return s;
@@ -283,7 +283,7 @@ __device__ inline void bp_tile3d(const BpLaunch &a, uint32_t bx, uint32_t by, ui
                 float w = s.w;
                 if constexpr (KIND == GCT_CYLINDRICAL) {
                     const float nvk = fmaf(kk, s.dnv, nvb);
-                    w = s.w * __builtin_amdgcn_rcpf(fmaf(nvk, nvk, s.rho2));
+                    w = __builtin_amdgcn_rcpf(fmaf(nvk, nvk, s.rho2));
                 }
                 acc[k] = fmaf(val, w, acc[k]);
             }
@@ -293,7 +293,7 @@ __device__ inline void bp_tile3d(const BpLaunch &a, uint32_t bx, uint32_t by, ui
                 const ColSetup s = setup_column<KIND, F64>(a, v, fx, fy, fz + static_cast<float>(k), nuf);
                 const float val = bilinear<S>(p, a.pitch, col_sample(s.col, nup), s.row0, nvp);
                 float w = s.w;
-                if constexpr (KIND == GCT_CYLINDRICAL) w = s.w * __builtin_amdgcn_rcpf(fmaf(s.nv0, s.nv0, s.rho2));
+                if constexpr (KIND == GCT_CYLINDRICAL) w = __builtin_amdgcn_rcpf(fmaf(s.nv0, s.nv0, s.rho2));
                 acc[k] = fmaf(val, w, acc[k]);
             }
         }
@@ -327,14 +327,14 @@ __device__ inline void bp_tile2d(const BpLaunch &a, uint32_t ix, uint32_t iy)
         } else if constexpr (KIND == GCT_FAN_LINE_1D) {
             const float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
             const float nu = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
-            const float rd = 1.0f / nw;
+            const float rd = __builtin_amdgcn_rcpf(nw);   // v_rcp_f32 (1 ulp), as for flat 3-D
             col = fmaf(nu, rd, r.k[0]);
             w = rd * rd;
         } else {   // GCT_FAN_ARC_1D
             const float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
             const float nu = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
             col = fmaf(atan2f(nu, nw), a.inv_dgamma, r.k[0]);
-            w = r.k[3] / fmaf(nu, nu, nw * nw);
+            w = __builtin_amdgcn_rcpf(fmaf(nu, nu, nw * nw));   // forms scaled by 1/sqrt(SID)
         }
         const float val = linear<S>(p, a.pitch, col_sample(col, nup));
         acc = fmaf(val, w, acc);
@@ -609,6 +609,10 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                         rowb[g] = row_base<KIND, F64>(s, zb);
                         nvb[g] = nv_base<KIND, F64>(s, zb);
                     }
+                    // the v step in a vector register, so that fma(k, dnv, base) takes k as a
+                    // literal (with a scalar dnv each k != 0, 1, 2, 4 needs a move first)
+                    float dnv = s.dnv;
+                    if constexpr (KIND == GCT_CYLINDRICAL) asm("" : "=v"(dnv) : "0"(dnv));
 #pragma unroll
                     for (int k = 0; k < ZU; ++k) {
                         if constexpr (WIN) {
@@ -628,8 +632,8 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                         const float val = fmaf(beta, bot - top, top);
                         float w = s.w;
                         if constexpr (KIND == GCT_CYLINDRICAL) {
-                            const float nvk = fmaf(kk, s.dnv, nvb[g]);
-                            w = s.w * __builtin_amdgcn_rcpf(fmaf(nvk, nvk, s.rho2));
+                            const float nvk = k % static_cast<int>(kZBlock) == 0 ? nvb[g] : fmaf(kk, dnv, nvb[g]);
+                            w = __builtin_amdgcn_rcpf(fmaf(nvk, nvk, s.rho2));
                         }
                         acc[k] = fmaf(val, w, acc[k]);
                     }
@@ -654,7 +658,7 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                         float w = s.w;
                         if constexpr (KIND == GCT_CYLINDRICAL) {
                             const float nvk = fmaf(kk, s.dnv, nvb[g]);
-                            w = s.w * __builtin_amdgcn_rcpf(fmaf(nvk, nvk, s.rho2));
+                            w = __builtin_amdgcn_rcpf(fmaf(nvk, nvk, s.rho2));
                         }
                         acc[k] = fmaf(val, w, acc[k]);
                     }
@@ -737,14 +741,14 @@ __global__ __launch_bounds__(256) void bp2d_lds_kernel(BpLaunch a, uint32_t vb)
                 } else if constexpr (KIND == GCT_FAN_LINE_1D) {
                     const float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
                     const float nu = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
-                    const float rd = 1.0f / nw;
+                    const float rd = __builtin_amdgcn_rcpf(nw);   // v_rcp_f32 (1 ulp), as for flat 3-D
                     col = fmaf(nu, rd, r.k[0]);
                     w = rd * rd;
                 } else {
                     const float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
                     const float nu = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
                     col = fmaf(atan2f(nu, nw), a.inv_dgamma, r.k[0]);
-                    w = r.k[3] / fmaf(nu, nu, nw * nw);
+                    w = __builtin_amdgcn_rcpf(fmaf(nu, nu, nw * nw));   // forms scaled by 1/sqrt(SID)
                 }
                 const Sample cs = col_sample(col, nup);
                 const float *rr = rows + j * a.pitch + cs.colbase;

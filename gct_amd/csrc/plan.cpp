@@ -148,15 +148,18 @@ void build_records(gct_plan &p, std::vector<BpRec> &rec, std::vector<BpRec64> &r
         case GCT_FAN_ARC_1D:
         case GCT_CYLINDRICAL: {
             // gamma = atan2(r.eu, r.ew); v = D (r^.ev)/sqrt(1 - (r^.ev)^2) = D (r.ev)/rho
-            // for an orthonormal basis; w = SID / |r|^2 (coord_trans_proj.cuh:80-105, 175-208)
+            // for an orthonormal basis; w = SID / |r|^2 (coord_trans_proj.cuh:80-105, 175-208).
+            // gamma and v are unchanged when the three forms are scaled together, so
+            // they are scaled by 1/sqrt(SID): the weight is then 1 / |r|^2 directly.
             const double sid = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
-            affine(s, ew, 1.0, r.w);
-            affine(s, eu, 1.0, r.u);
-            if (g.kind == GCT_CYLINDRICAL) affine(s, ev, 1.0, r.v);
+            const double c = 1.0 / std::sqrt(sid);
+            affine(s, ew, c, r.w);
+            affine(s, eu, c, r.u);
+            if (g.kind == GCT_CYLINDRICAL) affine(s, ev, c, r.v);
             r.k[0] = cu;
             r.k[1] = cv;
             r.k[2] = D / dv;
-            r.k[3] = sid;
+            r.k[3] = 1.0;
             break;
         }
         }
