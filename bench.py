@@ -456,7 +456,8 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {a.gpus} is not supported (the FDK path here "
-              "runs on one device; DESIGN.md describes how it would shard).", file=sys.stderr)
+              "runs on one device: independent scans and requests would shard one per GPU, a single "
+              "large volume by z slab).", file=sys.stderr)
         sys.exit(2)
     sections = {s.strip() for s in a.sections.split(",") if s.strip()}
 
