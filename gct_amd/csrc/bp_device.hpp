@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <type_traits>
 
 namespace gct {
 namespace {
@@ -581,8 +582,16 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
     if (vfirst < vend) tile_boxes<KIND, ZU>(a, bx, by, bz, vfirst, min(static_cast<uint32_t>(VB), vend - vfirst),
                                             TR, TP, boxes, tid);
     __syncthreads();
-    uint32_t cur = 0;
-    for (uint32_t vb0 = vfirst; vb0 < vend; vb0 += VB, cur ^= 1u) {
+    // One view block: stage, backproject, find the next block's boxes.  The
+    // fast form serves blocks whose boxes all fit the staging area and has no
+    // per-view branch; the general form reads a view whose box does not fit
+    // from HBM.  A tile runs the fast form until its first block with such a
+    // view and the general form from there on, so the accumulators are not
+    // merged from two branches per view (the compiler then copies all of them
+    // every view).  Both forms take the views in the same order.
+    uint32_t cur = 0, vb0 = vfirst;
+    auto view_block = [&](auto general) __attribute__((always_inline)) {
+        constexpr bool kGeneral = decltype(general)::value;
         const uint32_t nvb = min(static_cast<uint32_t>(VB), vend - vb0);
         const Box *bc = boxes + cur * VB;
         stage_block<S, TR, VB, TP>(base, view_elems, a.pitch, vb0, nvb, bc, tile, wave, lane);
@@ -592,7 +601,7 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                 const uint32_t v = vb0 + j;
                 const ColSetup s = setup_column<KIND, F64, false>(a, v, fx, fy, 0.0f, nuf);
                 const Box b = bc[j];
-                if (b.ncols != 0) {
+                if (!kGeneral || b.ncols != 0) {
                     // Padded coordinates, as in col_sample / bilinear.  The box clamps never
                     // bind for a coordinate of this tile, which the box covers with a margin.
                     const float clp = __builtin_amdgcn_fmed3f(s.col, b.c0, b.c1);
@@ -613,6 +622,10 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                     // literal (with a scalar dnv each k != 0, 1, 2, 4 needs a move first)
                     float dnv = s.dnv;
                     if constexpr (KIND == GCT_CYLINDRICAL) asm("" : "=v"(dnv) : "0"(dnv));
+                    // an opaque copy of the row step: the rows are formed here, one per
+                    // update, not hoisted above the branch (16 more live registers)
+                    float drow = s.drow;
+                    asm("" : "+v"(drow));
 #pragma unroll
                     for (int k = 0; k < ZU; ++k) {
                         if constexpr (WIN) {
@@ -621,7 +634,7 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                         const float kk = static_cast<float>(k % static_cast<int>(kZBlock));
                         const int g = k / static_cast<int>(kZBlock);
                         // fma(0, drow, base) is base: slice 0 of a block takes the base as is
-                        const float rk = k % static_cast<int>(kZBlock) == 0 ? rowb[g] : fmaf(kk, s.drow, rowb[g]);
+                        const float rk = k % static_cast<int>(kZBlock) == 0 ? rowb[g] : fmaf(kk, drow, rowb[g]);
                         const float row = __builtin_amdgcn_fmed3f(rk, rlo, rhi);
                         const float beta = __builtin_amdgcn_fractf(row);
                         const uint32_t off = __umul24(static_cast<uint32_t>(row), 8u * TP) + obase;
@@ -670,7 +683,20 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
             tile_boxes<KIND, ZU>(a, bx, by, bz, vn, min(static_cast<uint32_t>(VB), vend - vn), TR, TP,
                                  boxes + (cur ^ 1u) * VB, tid);
         __syncthreads();
+        vb0 = vn;
+        cur ^= 1u;
+    };
+    for (; vb0 < vend;) {
+        const uint32_t nvb = min(static_cast<uint32_t>(VB), vend - vb0);
+        const Box *bc = boxes + cur * VB;
+        bool fits = true;
+#pragma unroll
+        for (int j = 0; j < VB; ++j)
+            fits = fits && (static_cast<uint32_t>(j) >= nvb || bc[j].ncols != 0);
+        if (!__builtin_amdgcn_readfirstlane(fits)) break;
+        view_block(std::false_type{});
     }
+    for (; vb0 < vend;) view_block(std::true_type{});
     if (!active) return;
     const float sc = a.scale;
 #pragma unroll
