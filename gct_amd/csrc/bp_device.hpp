@@ -538,10 +538,11 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                               Box *boxes)
 {
     using T = typename Stor<S>::T;
-    // Each wave takes an 8 x 8 quadrant of the tile: its lanes' detector columns
-    // then span about half as many LDS banks per view as a 16 x 4 strip's do.
-    const uint32_t ix = bx * 16u + ((tid >> 6) & 1u) * 8u + (tid & 7u);
-    const uint32_t iy = by * 16u + (tid >> 7) * 8u + ((tid >> 3) & 7u), iz0 = bz * ZU;
+    // Each wave takes an 8 x 8 quadrant of the tile, and each 16 lanes of it a
+    // 4 x 4 block: the lanes' detector columns then span about a quarter as
+    // many LDS banks per view as a 16-voxel strip's do.
+    const uint32_t ix = bx * 16u + ((tid >> 6) & 1u) * 8u + ((tid >> 4) & 1u) * 4u + (tid & 3u);
+    const uint32_t iy = by * 16u + (tid >> 7) * 8u + ((tid >> 5) & 1u) * 4u + ((tid >> 2) & 3u), iz0 = bz * ZU;
     const bool active = ix < a.nx && iy < a.ny;
     const uint32_t zn = min(static_cast<uint32_t>(ZU), a.nz - iz0);
     const size_t slice = static_cast<size_t>(a.nx) * a.ny;
