@@ -406,7 +406,10 @@ struct alignas(16) Box {
     int ebase, pad0, pad1, pad2;
 };
 
-template <int KIND, int ZU>
+// PADZ: the tile's last z corner is taken past the volume's last slice, so
+// the box also holds the rows of the slices a partial z tile computes and
+// discards (needed where the kernel does not clamp rows).
+template <int KIND, int ZU, bool PADZ = false>
 __device__ inline void tile_boxes(const BpLaunch &a, uint32_t bx, uint32_t by, uint32_t bz, uint32_t vb0,
                                   uint32_t nvb, int TR, int TP, Box *boxes, uint32_t tid)
 {
@@ -414,7 +417,8 @@ __device__ inline void tile_boxes(const BpLaunch &a, uint32_t bx, uint32_t by, u
     const uint32_t j = tid >> 3, corner = tid & 7u;
     const uint32_t cx = min(bx * 16u + ((corner & 1u) ? 15u : 0u), a.nx - 1u);
     const uint32_t cy = min(by * 16u + ((corner & 2u) ? 15u : 0u), a.ny - 1u);
-    const uint32_t cz = min(bz * ZU + ((corner & 4u) ? static_cast<uint32_t>(ZU - 1) : 0u), a.nz - 1u);
+    const uint32_t czt = bz * ZU + ((corner & 4u) ? static_cast<uint32_t>(ZU - 1) : 0u);
+    const uint32_t cz = PADZ ? czt : min(czt, a.nz - 1u);
     const float nuf = static_cast<float>(a.nu);
     const ColSetup s = setup_column<KIND, false, false>(a, vb0 + j, static_cast<float>(cx), static_cast<float>(cy), 0.0f, nuf);
     const float row = fmaf(static_cast<float>(cz), s.drow, s.row0);
@@ -533,7 +537,10 @@ __device__ inline void stage_block(const typename Stor<S>::T *base, size_t view_
 // WIN: spiral scans -- slice z only takes the views [zwin[2z], zwin[2z+1]);
 // the tile's view loop covers the union of its slices' windows, and the
 // per-slice test is on wave-uniform values (scalar branches).
-template <int KIND, int S, bool F64, int ZU, int TR, int VB, int TP, bool WIN = false>
+// CLAMP = false: the plan showed that every voxel's row lies inside the
+// detector (BpLaunch::rows_inside), so each row lies inside its staged box
+// and the per-update row clamp is left out.
+template <int KIND, int S, bool F64, int ZU, int TR, int VB, int TP, bool WIN = false, bool CLAMP = true>
 __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint32_t bz, uint32_t tid, float2 *tile,
                               Box *boxes)
 {
@@ -583,8 +590,9 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
     // Two barriers per view block: the boxes of block b + 1 are found after
     // block b is backprojected, into the other of the two box slots, so they
     // are ready at the barrier that also frees the staging area.
-    if (vfirst < vend) tile_boxes<KIND, ZU>(a, bx, by, bz, vfirst, min(static_cast<uint32_t>(VB), vend - vfirst),
-                                            TR, TP, boxes, tid);
+    if (vfirst < vend)
+        tile_boxes<KIND, ZU, !CLAMP>(a, bx, by, bz, vfirst, min(static_cast<uint32_t>(VB), vend - vfirst), TR, TP,
+                                     boxes, tid);
     __syncthreads();
     // One view block: stage, backproject, find the next block's boxes.  The
     // fast form serves blocks whose boxes all fit the staging area and has no
@@ -639,7 +647,7 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                         const int g = k / static_cast<int>(kZBlock);
                         // fma(0, drow, base) is base: slice 0 of a block takes the base as is
                         const float rk = k % static_cast<int>(kZBlock) == 0 ? rowb[g] : fmaf(kk, drow, rowb[g]);
-                        const float row = __builtin_amdgcn_fmed3f(rk, rlo, rhi);
+                        const float row = CLAMP ? __builtin_amdgcn_fmed3f(rk, rlo, rhi) : rk;
                         const float beta = __builtin_amdgcn_fractf(row);
                         const uint32_t off = __umul24(static_cast<uint32_t>(row), 8u * TP) + obase;
                         const float2 *q = reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(tile) + off);
@@ -684,8 +692,8 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
         }
         const uint32_t vn = vb0 + VB;
         if (vn < vend)
-            tile_boxes<KIND, ZU>(a, bx, by, bz, vn, min(static_cast<uint32_t>(VB), vend - vn), TR, TP,
-                                 boxes + (cur ^ 1u) * VB, tid);
+            tile_boxes<KIND, ZU, !CLAMP>(a, bx, by, bz, vn, min(static_cast<uint32_t>(VB), vend - vn), TR, TP,
+                                         boxes + (cur ^ 1u) * VB, tid);
         __syncthreads();
         vb0 = vn;
         cur ^= 1u;
@@ -723,7 +731,7 @@ constexpr int lds_waves_per_simd()
     return by_lds < 1 ? 1 : (by_lds < by_regs ? by_lds : by_regs);
 }
 
-template <int KIND, int S, bool F64, int ZU, int TR, int VB, int TP, bool WIN = false>
+template <int KIND, int S, bool F64, int ZU, int TR, int VB, int TP, bool WIN = false, bool CLAMP = true>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(lds_waves_per_simd<ZU, TR, VB, TP>(), 8)))
 void bp3d_lds_kernel(BpLaunch a)
 {
@@ -732,7 +740,7 @@ void bp3d_lds_kernel(BpLaunch a)
     const uint32_t nbx = (a.nx + 15u) / 16u, nby = (a.ny + 15u) / 16u;
     const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t bx = t % nbx, rest = t / nbx;
-    bp_tile3d_lds<KIND, S, F64, ZU, TR, VB, TP, WIN>(a, bx, rest % nby, rest / nby, threadIdx.x, tile, boxes);
+    bp_tile3d_lds<KIND, S, F64, ZU, TR, VB, TP, WIN, CLAMP>(a, bx, rest % nby, rest / nby, threadIdx.x, tile, boxes);
 }
 
 // 2-D images: each block of views has its (padded) detector rows staged whole
@@ -861,6 +869,19 @@ void launch_bp3d_zu(const BpLaunch &a, hipStream_t s)
     }
 }
 
+template <int KIND, int S, bool F64, bool CLAMP>
+void launch_lds_shape(int shape, dim3 g, dim3 b, const BpLaunch &a, hipStream_t s)
+{
+    switch (shape) {
+    case 1: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 32, 2, 40, false, CLAMP>), g, b, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 2, 48, false, CLAMP>), g, b, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 28, 3, 34, false, CLAMP>), g, b, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 24, 4, 28, false, CLAMP>), g, b, 0, s, a); break;
+    case 5: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 32, 64, 2, 48, false, CLAMP>), g, b, 0, s, a); break;
+    default: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 2, 40, false, CLAMP>), g, b, 0, s, a); break;
+    }
+}
+
 template <int KIND, int S, bool F64>
 void launch_bp3d_lds(const BpLaunch &a, hipStream_t s)
 {
@@ -883,14 +904,8 @@ void launch_bp3d_lds(const BpLaunch &a, hipStream_t s)
     static_assert(kLdsShapes[3].zu == 16 && kLdsShapes[3].vb == 3 && kLdsShapes[3].rows == 28 && kLdsShapes[3].pairs == 34);
     static_assert(kLdsShapes[4].zu == 16 && kLdsShapes[4].vb == 4 && kLdsShapes[4].rows == 24 && kLdsShapes[4].pairs == 28);
     static_assert(kLdsShapes[5].zu == 32 && kLdsShapes[5].vb == 2 && kLdsShapes[5].rows == 64 && kLdsShapes[5].pairs == 48);
-    switch (shape) {
-    case 1: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 32, 2, 40>), g, b, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 2, 48>), g, b, 0, s, a); break;
-    case 3: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 28, 3, 34>), g, b, 0, s, a); break;
-    case 4: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 24, 4, 28>), g, b, 0, s, a); break;
-    case 5: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 32, 64, 2, 48>), g, b, 0, s, a); break;
-    default: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 2, 40>), g, b, 0, s, a); break;
-    }
+    if (a.rows_inside) launch_lds_shape<KIND, S, F64, false>(shape, g, b, a, s);
+    else launch_lds_shape<KIND, S, F64, true>(shape, g, b, a, s);
 }
 
 template <int KIND, int S>

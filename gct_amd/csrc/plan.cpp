@@ -176,6 +176,104 @@ void build_records(gct_plan &p, std::vector<BpRec> &rec, std::vector<BpRec64> &r
     }
 }
 
+// True when, for every view, every voxel of the volume -- z padded to whole
+// 32-slice tiles, the slices an LDS tile computes past the last -- projects
+// strictly between padded rows 1 and nv + 1, with a margin for the FP32 set-up.
+// Then the LDS kernel's staged box of a tile is never cut by the detector's
+// edge and contains the rows of all the tile's voxels (the box spans the rows
+// of the tile's corners with a margin of one row): its row clamp can never
+// bind, and the kernel is built without it.  In FP64 from the FP64 records:
+//   flat: row = nv / nw + k1 is linear-fractional; when nw has one sign over
+//     the volume, its extremes over any box are at corners;
+//   parallel kinds: the row is affine in z;
+//   cylindrical: row = k1 + k2 nv / rho with rho = |(nu, nw)|: bounded from
+//     nv's corner range and rho's range over the volume's footprint; a tile's
+//     corners can miss rho's minimum d over the tile (inside an edge) by at
+//     most sqrt(d^2 + h^2) - d <= h^2 / (2 d) (h: half a tile edge), which moves a
+//     row by at most |k2 nv| h^2 / (2 rho_min^3); twice that must stay below
+//     half a row, and it widens the bounds.
+bool rows_inside_detector(const gct_plan &p, const std::vector<BpRec64> &rec64)
+{
+    const gct_ct_geom &g = p.geom;
+    if (!p.circular || p.img.dim != 3 || is_1d(g.kind)) return false;
+    const double xm = double(p.img.x.num_samples) - 1.0, ym = double(p.img.y.num_samples) - 1.0;
+    const double zm = double((p.img.z.num_samples + 31u) / 32u * 32u) - 1.0;
+    const double lo = 1.05, hi = double(p.nv) + 0.95;
+    const double X[2] = {0.0, xm}, Y[2] = {0.0, ym}, Z[2] = {0.0, zm};
+    for (const BpRec64 &r : rec64) {
+        auto form = [](const double f[4], double x, double y, double z) { return f[0] + f[1] * x + f[2] * y + f[3] * z; };
+        double rmin = 1e300, rmax = -1e300;
+        if (g.kind == GCT_PARALLEL_2D || g.kind == GCT_CONE_PARALLEL) {
+            for (double z : Z) {
+                const double row = r.v[0] + r.v[3] * z;
+                rmin = std::min(rmin, row);
+                rmax = std::max(rmax, row);
+            }
+        } else if (g.kind == GCT_FLAT) {
+            double wmin = 1e300, wmax = -1e300;
+            for (double x : X)
+                for (double y : Y)
+                    for (double z : Z) {
+                        const double nw = form(r.w, x, y, z);
+                        wmin = std::min(wmin, nw);
+                        wmax = std::max(wmax, nw);
+                        const double row = form(r.v, x, y, z) / nw + r.k[1];
+                        rmin = std::min(rmin, row);
+                        rmax = std::max(rmax, row);
+                    }
+            if (!(wmin > 0.0 || wmax < 0.0)) return false;
+        } else if (g.kind == GCT_CYLINDRICAL) {
+            // (nu, nw) over the footprint: a parallelogram; rho's range over it
+            double pu[4], pw[4], nvmin = 1e300, nvmax = -1e300;
+            for (int c = 0; c < 4; ++c) {
+                const double x = X[c & 1], y = Y[c >> 1];
+                pu[c] = form(r.u, x, y, 0.0);
+                pw[c] = form(r.w, x, y, 0.0);
+            }
+            for (double x : X)
+                for (double y : Y)
+                    for (double z : Z) {
+                        const double nv = form(r.v, x, y, z);
+                        nvmin = std::min(nvmin, nv);
+                        nvmax = std::max(nvmax, nv);
+                    }
+            double rho_max = 0.0, rho_min = 1e300;
+            const int edge[4][2] = {{0, 1}, {1, 3}, {3, 2}, {2, 0}};
+            double cross_sign = 0.0;
+            bool inside = true;
+            for (const auto &e : edge) {
+                const double ax = pu[e[0]], ay = pw[e[0]], bx = pu[e[1]], by = pw[e[1]];
+                rho_max = std::max(rho_max, std::hypot(ax, ay));
+                const double ex = bx - ax, ey = by - ay, len2 = ex * ex + ey * ey;
+                const double t = len2 > 0.0 ? std::clamp(-(ax * ex + ay * ey) / len2, 0.0, 1.0) : 0.0;
+                rho_min = std::min(rho_min, std::hypot(ax + t * ex, ay + t * ey));
+                const double cr = ax * ey - ay * ex;   // origin on the same side of every edge: inside
+                if (cross_sign == 0.0) cross_sign = cr;
+                else if (cr * cross_sign <= 0.0) inside = false;
+            }
+            if (inside || !(rho_min > 0.0)) return false;
+            const double k2 = r.k[2];
+            const double q[4] = {nvmin / rho_min, nvmin / rho_max, nvmax / rho_min, nvmax / rho_max};
+            for (double v : q) {
+                rmin = std::min(rmin, r.k[1] + k2 * v);
+                rmax = std::max(rmax, r.k[1] + k2 * v);
+            }
+            // a tile edge in the forms' units: 15 voxels times the longer step
+            const double step = std::max(std::hypot(r.u[1], r.w[1]), std::hypot(r.u[2], r.w[2]));
+            const double h = 7.5 * step;
+            const double dev = std::fabs(k2) * std::max(std::fabs(nvmin), std::fabs(nvmax)) * h * h /
+                               (rho_min * rho_min * rho_min);
+            if (!(dev < 0.5)) return false;
+            rmin -= dev;
+            rmax += dev;
+        } else {
+            return false;
+        }
+        if (!(rmin >= lo && rmax <= hi)) return false;
+    }
+    return true;
+}
+
 // The arc/cylindrical mapping uses rho = |(r.eu, r.ew)|, which equals the
 // reference's |r| sqrt(1 - (r^.ev)^2) only for an orthonormal detector basis.
 void check_basis(const gct_plan &p)
@@ -263,6 +361,7 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
     std::vector<BpRec64> rec64;
     std::vector<FiltRec> frec;
     build_records(*p, rec, rec64, frec);
+    p->rows_inside = rows_inside_detector(*p, rec64);
     upload(*p, p->d_brec, rec.data(), rec.size() * sizeof(BpRec));
     upload(*p, p->d_brec64, rec64.data(), rec64.size() * sizeof(BpRec64));
     upload(*p, p->d_frec, frec.data(), frec.size() * sizeof(FiltRec));
@@ -412,6 +511,7 @@ BpLaunch bp_args(gct_plan &p, float *vol, uint32_t first, uint32_t n, bool accum
     a.view_block = p.view_block;
     a.lds_rows = p.lds_rows;
     a.lds_pairs = p.lds_pairs;
+    a.rows_inside = p.rows_inside;
     return a;
 }
 
@@ -564,6 +664,7 @@ extern "C" gct_status gct_plan_get_info(const gct_plan *p, gct_plan_info *info)
         info->view_block = lds ? kLdsShapes[shape].vb : 0u;
         info->lds_rows = lds ? kLdsShapes[shape].rows : 0u;
         info->lds_pairs = lds ? kLdsShapes[shape].pairs : 0u;
+        info->rows_inside = p->rows_inside;
     });
 }
 

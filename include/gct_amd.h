@@ -241,6 +241,8 @@ typedef struct gct_plan_info {
     uint32_t view_block;             /* LDS BP shape in use (0 for the direct kernel): views per block, */
     uint32_t lds_rows;               /*   staged rows per view, */
     uint32_t lds_pairs;              /*   pairs per staged row */
+    int32_t rows_inside;             /* 1 when every voxel projects inside the detector rows in every
+                                        view: the LDS backprojection then runs without its row clamp */
 } gct_plan_info;
 
 /* Builds per-view tables (FP64 on the host, stored FP32), the redundancy

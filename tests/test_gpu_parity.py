@@ -150,15 +150,22 @@ def test_one_shot_high_low_memory_identical(torch_dev, mode):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("nv", [80, 40], ids=["rows_inside", "rows_cut"])
 @pytest.mark.parametrize("setup", [g.SETUP_F32, g.SETUP_F64], ids=["f32", "f64"])
 @pytest.mark.parametrize("kind", KINDS, ids=g.KIND_NAMES)
-def test_lds_and_direct_kernels_identical(torch_dev, kind, setup):
+def test_lds_and_direct_kernels_identical(torch_dev, kind, setup, nv):
     """The LDS-staged and the direct backprojector read the same taps and run
     the same arithmetic per update, and every voxel's row coordinate is
     evaluated from its 16-slice z block, so they agree bit for bit whatever the
-    z unroll and LDS shape, with FP32 and with FP64 per-column set-up."""
+    z unroll and LDS shape, with FP32 and with FP64 per-column set-up.  With 80
+    detector rows the 64^3 volume projects inside the detector in every view
+    (the plan reports rows_inside and the LDS kernel runs without its row
+    clamp); with 40 rows it does not, the staged boxes are cut by the
+    detector's edges and the clamp reproduces the zero border."""
     torch = torch_dev
-    wl = W.small_case(kind)
+    if g.is_1d(kind) and nv != 80:
+        pytest.skip("2-D images have no detector rows")
+    wl = W.small_case(kind, nv=nv)
     proj = torch.from_numpy(wl.projections()).cuda()
     out = []
     for bk, zu, shape in [(g.BP_DIRECT, 8, None), (g.BP_DIRECT, 4, None)] + [(g.BP_LDS, sh[0], sh[1:]) for sh in LDS_SHAPES]:
@@ -172,12 +179,39 @@ def test_lds_and_direct_kernels_identical(torch_dev, kind, setup):
         if bk == g.BP_LDS and not g.is_1d(kind):
             i = plan.info()
             assert (i.z_unroll, i.view_block, i.lds_rows, i.lds_pairs) == (zu, vb, rows, pairs)
+        assert plan.info().rows_inside == (not g.is_1d(kind) and nv == 80)
         v = torch.empty(wl.img.shape, dtype=torch.float32, device="cuda")
         plan.execute(proj, v)
         torch.cuda.synchronize()
         out.append(v.cpu().numpy())
     for o in out[1:]:
         assert np.array_equal(o, out[0])
+
+
+@pytest.mark.parametrize("kind", [g.PARALLEL_2D, g.CONE_PARALLEL, g.CYLINDRICAL, g.FLAT],
+                         ids=["parallel_2d", "cone_parallel", "cylindrical", "flat"])
+def test_lds_without_row_clamp_partial_z_tile(torch_dev, kind):
+    """56 slices: the last z tile of every shape is partial, and the LDS kernel
+    without its row clamp computes (and discards) the rows of slices past the
+    volume's end; its boxes are found for the whole tile, so those rows stay
+    inside the staged box and the stored slices match the direct kernel's.
+    96 detector rows: the plan's check covers the slices up to the next
+    multiple of 32 (here 64), which must project inside the detector too."""
+    torch = torch_dev
+    wl = W.small_case(kind, n=56, nv=96)
+    proj = torch.from_numpy(wl.projections()).cuda()
+    ref = None
+    for bk, zu, (vb, rows, pairs) in [(g.BP_DIRECT, 8, (0, 0, 0))] + [(g.BP_LDS, sh[0], sh[1:]) for sh in LDS_SHAPES]:
+        plan = g.Plan(wl.scan, wl.img, bp_kernel=bk, z_unroll=zu, view_block=vb, lds_rows=rows, lds_pairs=pairs)
+        assert plan.info().rows_inside == 1
+        v = torch.empty(wl.img.shape, dtype=torch.float32, device="cuda")
+        plan.execute(proj, v)
+        torch.cuda.synchronize()
+        plan.close()
+        if ref is None:
+            ref = v
+        else:
+            assert torch.equal(v, ref), f"z_unroll {zu} shape {(vb, rows, pairs)}: {(v != ref).sum().item()} voxels differ"
 
 
 def test_lds_and_direct_kernels_identical_512(torch_dev):
@@ -192,7 +226,7 @@ def test_lds_and_direct_kernels_identical_512(torch_dev):
     for bk, zu, (vb, rows, pairs) in [(g.BP_DIRECT, 8, (0, 0, 0))] + [(g.BP_LDS, sh[0], sh[1:]) for sh in LDS_SHAPES]:
         plan = g.Plan(wl.scan, wl.img, window=wl.window, bp_kernel=bk, z_unroll=zu, view_block=vb, lds_rows=rows,
                       lds_pairs=pairs)
-        assert plan.info().setup_f64 == 1
+        assert plan.info().setup_f64 == 1 and plan.info().rows_inside == 1
         v = torch.empty(wl.img.shape, dtype=torch.float32, device="cuda")
         plan.execute(proj, v)
         torch.cuda.synchronize()
