@@ -457,43 +457,59 @@ __device__ inline void tile_boxes(const BpLaunch &a, uint32_t bx, uint32_t by, u
 }
 
 // Staging one view's box into LDS as (value, right neighbour - value) pairs.
-// The wave's rows are rw, rw + RSTEP, ...
-// (wave-uniform; with HALF, lanes 32-63 take the row below), so row guards are
-// scalar branches, load addresses are a uniform row pointer plus the lane's
-// offset, and the LDS stores take their row step as an immediate offset.  A
-// row of m pairs needs m + 1 values: each lane loads one and takes its right
-// neighbour from the next lane.
-template <int S, int RSTEP, int NIT, bool HALF>
-__device__ inline void stage_load(const typename Stor<S>::T *p, uint32_t pitch, int rw, int m, int nrows,
-                                  uint32_t lane, float *v)
+// The wave's rows are rw, rw + RSTEP, ... (wave-uniform; with HALF, lanes
+// 32-63 take the row below), so row guards are scalar branches.  The loads
+// are raw buffer loads over the view's bytes: the row's byte offset is a
+// scalar operand and the lane's column offset a vector operand fixed for the
+// view, so a row costs no address arithmetic, and a read past the view's end
+// returns 0 instead of faulting.  Each lane loads one value and takes its right
+// neighbour from the next lane (a row of m pairs needs m + 1 values).  Lanes up
+// to TP store, whatever the box's width m: pairs at columns m .. TP - 1 of a
+// row are never read (the clamped column keeps both taps below column m), and
+// a store mask fixed for the kernel costs no compare per row.
+template <int S>
+__device__ inline float stage_ld(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint32_t soff)
 {
-    using T = typename Stor<S>::T;
-    const int c = HALF ? static_cast<int>(lane & 31u) : static_cast<int>(lane);
-    const int h = HALF ? static_cast<int>(lane >> 5) : 0;
-    const uint32_t loff = static_cast<uint32_t>(h) * pitch + static_cast<uint32_t>(c);
+    if constexpr (S == GCT_STORE_F32) {
+        return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, static_cast<int>(voff), static_cast<int>(soff), 0));
+    } else {
+        const uint32_t h = __builtin_amdgcn_raw_buffer_load_b16(rs, static_cast<int>(voff), static_cast<int>(soff), 0);
+        if constexpr (S == GCT_STORE_F16)
+            return static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(h)));
+        else
+            return __uint_as_float(h << 16);
+    }
+}
+
+// Rows past the box are loaded from an offset past the view's end (0 back,
+// nothing fetched): no branch, and no value left undefined per row.
+template <int S, int RSTEP, int NIT, bool HALF>
+__device__ inline void stage_load(__amdgpu_buffer_rsrc_t rs, uint32_t row0, uint32_t row_bytes, uint32_t voff,
+                                  int rw, int nrows, float *v)
+{
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
         const int r = rw + it * RSTEP;
-        v[it] = 0.0f;
-        if (r < nrows) {
-            const T *pr = p + static_cast<size_t>(r) * pitch;
-            if (c <= m && r + h < nrows) v[it] = Stor<S>::ld(pr, loff);
-        }
+        const uint32_t soff = r < nrows ? row0 + static_cast<uint32_t>(r) * row_bytes : 0x80000000u;
+        v[it] = stage_ld<S>(rs, voff, soff);
     }
 }
 
 template <int TP, int RSTEP, int NIT, bool HALF>
-__device__ inline void stage_store(float2 *tl, int rw, int m, int nrows, uint32_t lane, const float *v)
+__device__ inline void stage_store(float2 *tl, int rw, int nrows, uint32_t lane, const float *v)
 {
-    const int c = HALF ? static_cast<int>(lane & 31u) : static_cast<int>(lane);
-    const int h = HALF ? static_cast<int>(lane >> 5) : 0;
-    float2 *tw = tl + (rw + h) * TP + c;
+    const uint32_t c = HALF ? (lane & 31u) : lane;
+    const uint32_t h = HALF ? (lane >> 5) : 0u;
+    const bool ok_all = c < static_cast<uint32_t>(TP);   // both rows of a HALF step are in the box
+    const bool ok_top = ok_all && h == 0u;              // only the upper one is
+    float2 *tw = tl + (static_cast<uint32_t>(rw) + h) * TP + c;
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
         const int r = rw + it * RSTEP;
         if (r < nrows) {
             const float d = next_lane(v[it]) - v[it];
-            if (c < m && r + h < nrows) tw[it * RSTEP * TP] = make_float2(v[it], d);
+            const bool both = !HALF || r + 1 < nrows;
+            if (both ? ok_all : ok_top) tw[it * RSTEP * TP] = make_float2(v[it], d);
         }
     }
 }
@@ -506,26 +522,33 @@ template <int S, int TR, int VB, int TP>
 __device__ inline void stage_block(const typename Stor<S>::T *base, size_t view_elems, uint32_t pitch, uint32_t vb0,
                                    uint32_t nvb, const Box *bx, float2 *tile, int wave, uint32_t lane)
 {
+    using T = typename Stor<S>::T;
+    const uint32_t view_bytes = static_cast<uint32_t>(view_elems * sizeof(T));   // < 2^32 (checked by the plan)
+    const uint32_t row_bytes = pitch * static_cast<uint32_t>(sizeof(T));
     for (uint32_t j = 0; j < nvb; ++j) {
         const int ncols = __builtin_amdgcn_readfirstlane(bx[j].ncols);
         if (ncols == 0) continue;
         const int clo = __builtin_amdgcn_readfirstlane(bx[j].clo);
         const int rlo = __builtin_amdgcn_readfirstlane(bx[j].rlo);
         const int nrows = __builtin_amdgcn_readfirstlane(bx[j].nrows);
-        const typename Stor<S>::T *p = base + static_cast<size_t>(vb0 + j) * view_elems +
-                                       (static_cast<size_t>(rlo + 1) * pitch + static_cast<uint32_t>(clo + 1));
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<T *>(base + static_cast<size_t>(vb0 + j) * view_elems), 0, static_cast<int>(view_bytes),
+            0x00020000);
+        const uint32_t row0 = (static_cast<uint32_t>(rlo + 1) * pitch + static_cast<uint32_t>(clo + 1)) *
+                              static_cast<uint32_t>(sizeof(T));
         float2 *tl = tile + j * (TR * TP);
-        const int m = ncols - 1;
-        if (m < 32) {
+        if (ncols - 1 < 32) {
             constexpr int kN = (TR + 7) / 8;
             float v[kN];
-            stage_load<S, 8, kN, true>(p, pitch, 2 * wave, m, nrows, lane, v);
-            stage_store<TP, 8, kN, true>(tl, 2 * wave, m, nrows, lane, v);
+            const uint32_t voff = ((lane >> 5) * pitch + (lane & 31u)) * static_cast<uint32_t>(sizeof(T));
+            stage_load<S, 8, kN, true>(rs, row0, row_bytes, voff, 2 * wave, nrows, v);
+            stage_store<TP, 8, kN, true>(tl, 2 * wave, nrows, lane, v);
         } else {
             constexpr int kW = (TR + 3) / 4;
             float v[kW];
-            stage_load<S, 4, kW, false>(p, pitch, wave, m, nrows, lane, v);
-            stage_store<TP, 4, kW, false>(tl, wave, m, nrows, lane, v);
+            const uint32_t voff = lane * static_cast<uint32_t>(sizeof(T));
+            stage_load<S, 4, kW, false>(rs, row0, row_bytes, voff, wave, nrows, v);
+            stage_store<TP, 4, kW, false>(tl, wave, nrows, lane, v);
         }
     }
 }

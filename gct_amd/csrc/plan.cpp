@@ -316,6 +316,9 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
     validate_geom(geom);
     validate_image(geom, img);
     if (geom.det_row.num_samples > 2048u) fail(GCT_ERR_UNSUPPORTED, "more than 2048 detector columns");
+    // one padded filtered view (FP32: (Nv + 3) x pitch x 4 B) is addressed by 31-bit buffer offsets
+    if (!is_1d(geom.kind) && (uint64_t(geom.det_col.num_samples) + 3u) * 2064u * 4u >= (uint64_t(1) << 31))
+        fail(GCT_ERR_UNSUPPORTED, "more than 260000 detector rows");
     if (o.storage < GCT_STORE_F32 || o.storage > GCT_STORE_BF16) fail(GCT_ERR_INVALID_ARGUMENT, "unknown storage");
     if (o.bp_kernel < GCT_BP_AUTO || o.bp_kernel > GCT_BP_LDS) fail(GCT_ERR_INVALID_ARGUMENT, "unknown bp_kernel");
     // 2-D images default to the direct kernel: the LDS one stages whole
