@@ -99,11 +99,13 @@ def cpu_baseline(wl, proj, gpu_vol, nthreads):
 def _bp_kernel_name(g, info):
     """rocprofv3's name of the backprojection kernel a flat-detector FP32 plan
     launches (template arguments: kind, storage, FP64 set-up, z unroll, and
-    for the LDS kernel staged rows, views per block, pairs per row)."""
+    for the LDS kernel staged rows, views per block, pairs per row, spiral
+    windows, row clamp -- off when every voxel projects inside the detector)."""
     f64 = "true" if info.setup_f64 else "false"
     if info.bp_kernel == g.BP_LDS:
+        clamp = "false" if info.rows_inside else "true"
         return (f"void gct::(anonymous namespace)::bp3d_lds_kernel<6, 0, {f64}, {info.z_unroll}, {info.lds_rows}, "
-                f"{info.view_block}, {info.lds_pairs}, false>(gct::BpLaunch)")
+                f"{info.view_block}, {info.lds_pairs}, false, {clamp}>(gct::BpLaunch)")
     return f"void gct::(anonymous namespace)::bp3d_kernel<6, 0, true, {f64}, {info.z_unroll}>(gct::BpLaunch)"
 
 

@@ -741,16 +741,16 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
 
 // Occupancy target of an LDS kernel variant: workgroups (one wave per SIMD
 // each) that fit the 160 KiB of LDS per CU, capped by what its registers
-// reach (z-unroll 16: 7, i.e. <= 72 VGPRs, which the compiler meets with a
-// few spills; 32: 3).  More resident workgroups hide more of the staging
-// latency: measured, the 6- and 7-workgroup shapes beat the 5-workgroup one
-// despite their spills.
+// reach (z-unroll 16: 7, i.e. <= 72 VGPRs; 32: 4, <= 128).  More resident
+// workgroups hide more of the staging latency: measured, the 6- and
+// 7-workgroup shapes of z-unroll 16 beat the 5-workgroup one; z-unroll 32
+// halves the per-view column set-up per update at 4.
 template <int ZU, int TR, int VB, int TP>
 constexpr int lds_waves_per_simd()
 {
     constexpr int bytes = VB * TR * TP * 8 + 2 * VB * static_cast<int>(sizeof(Box));
     constexpr int by_lds = (160 * 1024) / bytes;
-    constexpr int by_regs = ZU >= 32 ? 3 : 7;
+    constexpr int by_regs = ZU >= 32 ? 4 : 7;
     return by_lds < 1 ? 1 : (by_lds < by_regs ? by_lds : by_regs);
 }
 
@@ -900,7 +900,7 @@ void launch_lds_shape(int shape, dim3 g, dim3 b, const BpLaunch &a, hipStream_t 
     case 2: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 2, 48, false, CLAMP>), g, b, 0, s, a); break;
     case 3: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 28, 3, 34, false, CLAMP>), g, b, 0, s, a); break;
     case 4: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 24, 4, 28, false, CLAMP>), g, b, 0, s, a); break;
-    case 5: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 32, 64, 2, 48, false, CLAMP>), g, b, 0, s, a); break;
+    case 5: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 32, 48, 3, 34, false, CLAMP>), g, b, 0, s, a); break;
     default: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 2, 40, false, CLAMP>), g, b, 0, s, a); break;
     }
 }
@@ -926,7 +926,7 @@ void launch_bp3d_lds(const BpLaunch &a, hipStream_t s)
     static_assert(kLdsShapes[2].zu == 16 && kLdsShapes[2].vb == 2 && kLdsShapes[2].rows == 40 && kLdsShapes[2].pairs == 48);
     static_assert(kLdsShapes[3].zu == 16 && kLdsShapes[3].vb == 3 && kLdsShapes[3].rows == 28 && kLdsShapes[3].pairs == 34);
     static_assert(kLdsShapes[4].zu == 16 && kLdsShapes[4].vb == 4 && kLdsShapes[4].rows == 24 && kLdsShapes[4].pairs == 28);
-    static_assert(kLdsShapes[5].zu == 32 && kLdsShapes[5].vb == 2 && kLdsShapes[5].rows == 64 && kLdsShapes[5].pairs == 48);
+    static_assert(kLdsShapes[5].zu == 32 && kLdsShapes[5].vb == 3 && kLdsShapes[5].rows == 48 && kLdsShapes[5].pairs == 34);
     if (a.rows_inside) launch_lds_shape<KIND, S, F64, false>(shape, g, b, a, s);
     else launch_lds_shape<KIND, S, F64, true>(shape, g, b, a, s);
 }

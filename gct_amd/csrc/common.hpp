@@ -153,15 +153,16 @@ struct BpLaunch {
 // staged per block, staged rows per view and (value, difference) pairs per
 // staged row.  The LDS a workgroup stages (8 B per pair) sets how many
 // workgroups share a CU: 6 for the first shape, 7 for the second, fourth and
-// fifth, 5 for the third (boxes up to 48 pairs wide), 3 for the z-unroll-32
-// shape.  Deeper view blocks (fewer barriers per view) fit only small boxes:
-// up to 34 pairs by 28 rows for three views, 28 by 24 for four.  The first
-// shape of each z unroll is its default; the autotuner times them all.
+// fifth, 5 for the third (boxes up to 48 pairs wide), 4 for the z-unroll-32
+// shape (its registers allow no more).  Deeper view blocks (fewer barriers per
+// view) fit only small boxes: up to 34 pairs by 28 rows for three views of 16
+// slices, 28 by 24 for four; 32 slices span up to 48 rows.  The first shape
+// of each z unroll is its default; the autotuner times them all.
 struct LdsShape {
     uint32_t zu, vb, rows, pairs;
 };
 constexpr LdsShape kLdsShapes[] = {{16, 2, 40, 40}, {16, 2, 32, 40}, {16, 2, 40, 48},
-                                   {16, 3, 28, 34}, {16, 4, 24, 28}, {32, 2, 64, 48}};
+                                   {16, 3, 28, 34}, {16, 4, 24, 28}, {32, 3, 48, 34}};
 constexpr int kNumLdsShapes = static_cast<int>(sizeof(kLdsShapes) / sizeof(kLdsShapes[0]));
 
 // The first shape of z unroll `zu` (0: 16) that agrees with every nonzero
