@@ -232,6 +232,12 @@ __device__ inline float nv_base(const ColSetup &s, float zb)
 // fma per kZBlock slices besides one per slice.
 constexpr uint32_t kZBlock = 16;
 
+// Slices whose LDS taps the LDS backprojection reads before using the first:
+// at z unroll 32 (4 resident waves, up to 128 VGPRs) four, at 16 (7 waves,
+// up to 72) two -- more spill.  Cylindrical kernels, whose per-update weight
+// holds more registers, read one at a time.
+constexpr int kLdsGroup = 4;
+
 // Whether the row axis of a view stays parallel to z: then u, w and the
 // weight are constant along a voxel column and only the row coordinate moves.
 template <int KIND, bool CIRC>
@@ -661,29 +667,51 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                     // update, not hoisted above the branch (16 more live registers)
                     float drow = s.drow;
                     asm("" : "+v"(drow));
+                    // G slices at a time: their LDS reads are issued before the first is
+                    // used, so a wave keeps several in flight
+                    constexpr int G = (WIN || KIND == GCT_CYLINDRICAL) ? 1 : (ZU >= 32 ? kLdsGroup : 2);
 #pragma unroll
-                    for (int k = 0; k < ZU; ++k) {
-                        if constexpr (WIN) {
-                            if (v < wl[k] || v >= wh[k]) continue;
+                    for (int k0 = 0; k0 < ZU; k0 += G) {
+                        float2 tp[G], bt[G];
+                        float beta[G];
+#pragma unroll
+                        for (int i = 0; i < G; ++i) {
+                            const int k = k0 + i;
+                            if constexpr (WIN) {
+                                if (v < wl[k] || v >= wh[k]) continue;
+                            }
+                            const float kk = static_cast<float>(k % static_cast<int>(kZBlock));
+                            const int g = k / static_cast<int>(kZBlock);
+                            // fma(0, drow, base) is base: slice 0 of a block takes the base as is
+                            const float rk = k % static_cast<int>(kZBlock) == 0 ? rowb[g] : fmaf(kk, drow, rowb[g]);
+                            const float row = CLAMP ? __builtin_amdgcn_fmed3f(rk, rlo, rhi) : rk;
+                            beta[i] = __builtin_amdgcn_fractf(row);
+                            const uint32_t off = __umul24(static_cast<uint32_t>(row), 8u * TP) + obase;
+                            const float2 *q = reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(tile) + off);
+                            tp[i] = q[0];
+                            bt[i] = q[TP];
                         }
-                        const float kk = static_cast<float>(k % static_cast<int>(kZBlock));
-                        const int g = k / static_cast<int>(kZBlock);
-                        // fma(0, drow, base) is base: slice 0 of a block takes the base as is
-                        const float rk = k % static_cast<int>(kZBlock) == 0 ? rowb[g] : fmaf(kk, drow, rowb[g]);
-                        const float row = CLAMP ? __builtin_amdgcn_fmed3f(rk, rlo, rhi) : rk;
-                        const float beta = __builtin_amdgcn_fractf(row);
-                        const uint32_t off = __umul24(static_cast<uint32_t>(row), 8u * TP) + obase;
-                        const float2 *q = reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(tile) + off);
-                        const float2 tp = q[0], bt = q[TP];
-                        const float top = fmaf(alpha, tp.y, tp.x);
-                        const float bot = fmaf(alpha, bt.y, bt.x);
-                        const float val = fmaf(beta, bot - top, top);
-                        float w = s.w;
-                        if constexpr (KIND == GCT_CYLINDRICAL) {
-                            const float nvk = k % static_cast<int>(kZBlock) == 0 ? nvb[g] : fmaf(kk, dnv, nvb[g]);
-                            w = __builtin_amdgcn_rcpf(fmaf(nvk, nvk, s.rho2));
+                        // keep the group's reads ahead of its arithmetic (the scheduler
+                        // otherwise moves each read next to its use)
+                        if constexpr (G > 1) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int i = 0; i < G; ++i) {
+                            const int k = k0 + i;
+                            if constexpr (WIN) {
+                                if (v < wl[k] || v >= wh[k]) continue;
+                            }
+                            const float kk = static_cast<float>(k % static_cast<int>(kZBlock));
+                            const int g = k / static_cast<int>(kZBlock);
+                            const float top = fmaf(alpha, tp[i].y, tp[i].x);
+                            const float bot = fmaf(alpha, bt[i].y, bt[i].x);
+                            const float val = fmaf(beta[i], bot - top, top);
+                            float w = s.w;
+                            if constexpr (KIND == GCT_CYLINDRICAL) {
+                                const float nvk = k % static_cast<int>(kZBlock) == 0 ? nvb[g] : fmaf(kk, dnv, nvb[g]);
+                                w = __builtin_amdgcn_rcpf(fmaf(nvk, nvk, s.rho2));
+                            }
+                            acc[k] = fmaf(val, w, acc[k]);
                         }
-                        acc[k] = fmaf(val, w, acc[k]);
                     }
                 } else {
                     const T *p = base + static_cast<size_t>(v) * view_elems;
@@ -901,6 +929,7 @@ void launch_lds_shape(int shape, dim3 g, dim3 b, const BpLaunch &a, hipStream_t 
     case 3: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 28, 3, 34, false, CLAMP>), g, b, 0, s, a); break;
     case 4: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 24, 4, 28, false, CLAMP>), g, b, 0, s, a); break;
     case 5: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 32, 48, 3, 34, false, CLAMP>), g, b, 0, s, a); break;
+    case 6: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 32, 36, 4, 28, false, CLAMP>), g, b, 0, s, a); break;
     default: hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 2, 40, false, CLAMP>), g, b, 0, s, a); break;
     }
 }
@@ -927,6 +956,8 @@ void launch_bp3d_lds(const BpLaunch &a, hipStream_t s)
     static_assert(kLdsShapes[3].zu == 16 && kLdsShapes[3].vb == 3 && kLdsShapes[3].rows == 28 && kLdsShapes[3].pairs == 34);
     static_assert(kLdsShapes[4].zu == 16 && kLdsShapes[4].vb == 4 && kLdsShapes[4].rows == 24 && kLdsShapes[4].pairs == 28);
     static_assert(kLdsShapes[5].zu == 32 && kLdsShapes[5].vb == 3 && kLdsShapes[5].rows == 48 && kLdsShapes[5].pairs == 34);
+    static_assert(kLdsShapes[6].zu == 32 && kLdsShapes[6].vb == 4 && kLdsShapes[6].rows == 36 && kLdsShapes[6].pairs == 28);
+    static_assert(kNumLdsShapes == 7);
     if (a.rows_inside) launch_lds_shape<KIND, S, F64, false>(shape, g, b, a, s);
     else launch_lds_shape<KIND, S, F64, true>(shape, g, b, a, s);
 }

@@ -154,15 +154,18 @@ struct BpLaunch {
 // staged row.  The LDS a workgroup stages (8 B per pair) sets how many
 // workgroups share a CU: 6 for the first shape, 7 for the second, fourth and
 // fifth, 5 for the third (boxes up to 48 pairs wide), 4 for the z-unroll-32
-// shape (its registers allow no more).  Deeper view blocks (fewer barriers per
-// view) fit only small boxes: up to 34 pairs by 28 rows for three views of 16
-// slices, 28 by 24 for four; 32 slices span up to 48 rows.  The first shape
-// of each z unroll is its default; the autotuner times them all.
+// shapes (their registers allow no more).  Deeper view blocks (fewer barriers
+// per view) fit only small boxes: up to 34 pairs by 28 rows for three views of
+// 16 slices, 28 by 24 for four; 32 slices span up to 48 rows on a 2x
+// magnified flat cone, 34 on a 1.9x C-arm (whose boxes the last shape fits).
+// The first shape of each z unroll is its default; the autotuner times them
+// all.
 struct LdsShape {
     uint32_t zu, vb, rows, pairs;
 };
 constexpr LdsShape kLdsShapes[] = {{16, 2, 40, 40}, {16, 2, 32, 40}, {16, 2, 40, 48},
-                                   {16, 3, 28, 34}, {16, 4, 24, 28}, {32, 3, 48, 34}};
+                                   {16, 3, 28, 34}, {16, 4, 24, 28}, {32, 3, 48, 34},
+                                   {32, 4, 36, 28}};
 constexpr int kNumLdsShapes = static_cast<int>(sizeof(kLdsShapes) / sizeof(kLdsShapes[0]));
 
 // The first shape of z unroll `zu` (0: 16) that agrees with every nonzero

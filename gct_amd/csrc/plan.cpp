@@ -333,7 +333,7 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
         lds_shape(o.z_unroll, o.view_block, o.lds_rows, o.lds_pairs) < 0)
         fail(GCT_ERR_INVALID_ARGUMENT,
              "no LDS backprojection shape matches (z_unroll, view_block, lds_rows, lds_pairs); the shapes are "
-             "(16, 2, 40, 40), (16, 2, 32, 40), (16, 2, 40, 48), (16, 3, 28, 34), (16, 4, 24, 28), (32, 3, 48, 34)");
+             "(16, 2, 40, 40), (16, 2, 32, 40), (16, 2, 40, 48), (16, 3, 28, 34), (16, 4, 24, 28), (32, 3, 48, 34), (32, 4, 36, 28)");
     auto p = std::make_unique<gct_plan>();
     p->device = o.device;
     p->stream = stream;

@@ -141,7 +141,8 @@ typedef struct gct_recon_opts {
     uint32_t view_chunk;             /* LOW mode views per chunk; 0 = derived */
     int32_t bp_kernel;               /* gct_bp_kernel, default AUTO */
     /* LDS backprojection shape (z_unroll, view_block, lds_rows, lds_pairs): one of
-     * (16, 2, 40, 40), (16, 2, 32, 40), (16, 2, 40, 48), (16, 3, 28, 34), (16, 4, 24, 28), (32, 3, 48, 34);
+     * (16, 2, 40, 40), (16, 2, 32, 40), (16, 2, 40, 48), (16, 3, 28, 34), (16, 4, 24, 28), (32, 3, 48, 34),
+     * (32, 4, 36, 28);
      * a zero field matches any value, all zero = the first shape of z_unroll (or the tuned one) */
     uint32_t view_block;             /* views staged per block */
     uint32_t lds_rows;               /* detector rows staged per view */

@@ -35,7 +35,8 @@ def _report(name, got, want):
 
 KINDS = [g.PARALLEL_1D, g.FAN_ARC_1D, g.FAN_LINE_1D, g.PARALLEL_2D, g.CONE_PARALLEL, g.CYLINDRICAL, g.FLAT]
 # the LDS backprojection shapes (z unroll, views per block, staged rows, pairs per row)
-LDS_SHAPES = [(16, 2, 40, 40), (16, 2, 32, 40), (16, 2, 40, 48), (16, 3, 28, 34), (16, 4, 24, 28), (32, 3, 48, 34)]
+LDS_SHAPES = [(16, 2, 40, 40), (16, 2, 32, 40), (16, 2, 40, 48), (16, 3, 28, 34), (16, 4, 24, 28), (32, 3, 48, 34),
+              (32, 4, 36, 28)]
 
 
 @pytest.mark.parametrize("kind", KINDS, ids=g.KIND_NAMES)
@@ -243,7 +244,7 @@ def test_lds_shape_options(torch_dev):
     wl = W.small_case(g.FLAT)
     cases = [(dict(), (16, 2, 40, 40)), (dict(z_unroll=32), (32, 3, 48, 34)),
              (dict(lds_rows=32), (16, 2, 32, 40)), (dict(lds_pairs=48), (16, 2, 40, 48)),
-             (dict(view_block=3), (16, 3, 28, 34))]
+             (dict(view_block=3), (16, 3, 28, 34)), (dict(z_unroll=32, view_block=4), (32, 4, 36, 28))]
     for kw, want in cases:
         plan = g.Plan(wl.scan, wl.img, bp_kernel=g.BP_LDS, **kw)
         i = plan.info()
