@@ -45,6 +45,11 @@ uint32_t next_pow2(uint32_t n)
     return v;
 }
 
+uint32_t fft_len(uint32_t n)
+{
+    return std::max(8u, next_pow2(n + n / 2u));
+}
+
 RotationInfo rotation_info(const gct_ct_geom &g)
 {
     RotationInfo r;

@@ -260,7 +260,9 @@ constexpr uint32_t kFiltBlock = 256;
 // z = a + i b: IFFT(FFT(z) H) = (a * h) + i (b * h) because the kernel h is
 // real.  H is the FFT of the N-tap kernel circularly shifted by floor(N/2)
 // and scaled by 1/L, so output sample c is the reference's y[c + floor(N/2)] / L
-// (ramp_filtering_fft.cu:45-62).  The inverse FFT is conj(FFT(conj(.))).
+// (ramp_filtering_fft.cu:45-62): the truncated linear convolution, which any
+// L >= N + floor(N/2) gives without wrap-around (fft_len; the reference's L
+// is nextPow2(2 N)).  The inverse FFT is conj(FFT(conj(.))).
 template <int KIND, int S, uint32_t L>
 __global__ __launch_bounds__(kFiltBlock) void filter_kernel(FilterLaunch a)
 {
@@ -285,13 +287,14 @@ __global__ __launch_bounds__(kFiltBlock) void filter_kernel(FilterLaunch a)
     const float *red1 = a.red + static_cast<size_t>(v1) * nu;
 
     // Forward FFT: the first pass loads the weighted rows straight from global
-    // memory (points at or beyond L/2 >= nu are zero, so half its inputs are
-    // constant zeros); the last pass multiplies by the ramp spectrum and
-    // conjugates before writing LDS.
+    // memory (points at or beyond kMaxN >= nu are zero: about a third of its
+    // inputs are constant zeros); the last pass multiplies by the ramp spectrum
+    // and conjugates before writing LDS.
+    constexpr uint32_t kMaxN = (2u * L + 1u) / 3u;   // the widest row fft_len gives L for
     const float du = a.du, dv = a.dv;
     auto ld_rows = [&](uint32_t i, uint32_t j) -> float2 {
         constexpr uint32_t R0 = next_radix<L>();
-        if (j >= R0 / 2u) return make_float2(0.0f, 0.0f);
+        if (j * (L / R0) >= kMaxN) return make_float2(0.0f, 0.0f);   // j is a constant: no branch
         const uint32_t c = i + j * (L / R0);
         const bool in = c < nu && has0;
         const float xa = in ? pa[c] : 0.0f, wa = in ? red0[c] : 0.0f;
@@ -317,7 +320,7 @@ __global__ __launch_bounds__(kFiltBlock) void filter_kernel(FilterLaunch a)
     typename St::T *ob = out + static_cast<size_t>(v1) * view_elems + static_cast<size_t>(j1 + 1u) * a.pitch + 1u;
     auto st_rows = [&](uint32_t idx, uint32_t j, float2 z) {
         constexpr uint32_t RL = last_radix<L>();
-        if (j >= RL / 2u) return;   // idx >= L/2 >= nu
+        if (j * (L / RL) >= kMaxN) return;   // idx >= j L / RL >= kMaxN >= nu
         if (idx < nu && has0) {
             oa[idx] = St::cv(z.x);
             if (has1) ob[idx] = St::cv(-z.y);

@@ -315,10 +315,11 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
 {
     validate_geom(geom);
     validate_image(geom, img);
-    if (geom.det_row.num_samples > 2048u) fail(GCT_ERR_UNSUPPORTED, "more than 2048 detector columns");
+    if (geom.det_row.num_samples > kMaxDetCols) fail(GCT_ERR_UNSUPPORTED, "more than 2730 detector columns");
     // one padded filtered view (FP32: (Nv + 3) x pitch x 4 B) is addressed by 31-bit buffer offsets
-    if (!is_1d(geom.kind) && (uint64_t(geom.det_col.num_samples) + 3u) * 2064u * 4u >= (uint64_t(1) << 31))
-        fail(GCT_ERR_UNSUPPORTED, "more than 260000 detector rows");
+    if (!is_1d(geom.kind) &&
+        (uint64_t(geom.det_col.num_samples) + 3u) * ((kMaxDetCols + 3u + 15u) & ~15u) * 4u >= (uint64_t(1) << 31))
+        fail(GCT_ERR_UNSUPPORTED, "more than 196000 detector rows");
     if (o.storage < GCT_STORE_F32 || o.storage > GCT_STORE_BF16) fail(GCT_ERR_INVALID_ARGUMENT, "unknown storage");
     if (o.bp_kernel < GCT_BP_AUTO || o.bp_kernel > GCT_BP_LDS) fail(GCT_ERR_INVALID_ARGUMENT, "unknown bp_kernel");
     // 2-D images default to the direct kernel: the LDS one stages whole
@@ -346,7 +347,7 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
     p->V = geom.num_proj;
     p->nu = geom.det_row.num_samples;
     p->nv = is_1d(geom.kind) ? 1u : geom.det_col.num_samples;
-    p->L = next_pow2(2u * p->nu);
+    p->L = fft_len(p->nu);
     p->rot = rotation_info(p->geom);   // throws for an invalid trajectory (ct_geom.h:26-32)
     p->scale = (p->rot.incr > 0.0f) ? p->rot.incr : 1.0f;   // fdk_axial.cuh:196-199
     p->inv_dgamma = static_cast<float>(1.0 / double(geom.det_row.sampling_width));
@@ -560,7 +561,7 @@ uint64_t plan_device_bytes(const gct_ct_geom &g, const gct_image_geom &img, cons
     const uint64_t nu = g.det_row.num_samples, nv = is_1d(g.kind) ? 1u : g.det_col.num_samples;
     const uint64_t pitch = (nu + 3u + 15u) & ~uint64_t(15);
     const uint64_t V = g.num_proj;
-    const uint64_t L = next_pow2(uint32_t(2u * nu));
+    const uint64_t L = fft_len(uint32_t(nu));
     uint64_t b = uint64_t(fviews ? fviews : V) * (nv + 3u) * pitch * storage_bytes(o.storage);
     b += V * (sizeof(BpRec) + sizeof(BpRec64) + sizeof(FiltRec) + sizeof(gct_proj_geom)) + V * nu * 4u + 2u * L * 8u;
     (void)img;

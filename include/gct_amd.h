@@ -225,7 +225,9 @@ gct_status gct_required_memory(const gct_ct_geom *geom, const gct_image_geom *im
 typedef struct gct_plan gct_plan;
 
 typedef struct gct_plan_info {
-    uint32_t fft_len;                /* nextPow2(2 N) (ramp_filtering_fft.cuh:36-48) */
+    uint32_t fft_len;                /* nextPow2(N + floor(N/2)), at least 8: the shortest FFT whose circular
+                                        convolution leaves the N kept outputs unwrapped (the reference's
+                                        nextPow2(2 N), ramp_filtering_fft.cuh:36-48, gives the same values) */
     uint32_t filt_pitch;             /* elements per padded filtered row */
     uint32_t filt_rows_per_view;     /* Nv + 3 (one zero row above, two below) */
     uint32_t z_unroll;

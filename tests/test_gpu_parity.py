@@ -73,14 +73,45 @@ def test_parallel_1d_256_parity(torch_dev):
     assert O.are_equal(got, want, EPS).all()
 
 
-@pytest.mark.parametrize("kind", [g.PARALLEL_1D, g.FAN_ARC_1D, g.CONE_PARALLEL, g.FLAT], ids=lambda k: g.KIND_NAMES[k])
-def test_filter_stage_parity(torch_dev, kind):
-    """Fused cosine + redundancy weighting + FFT ramp filter against the oracle's
-    weights and exact truncated convolution."""
+@pytest.mark.parametrize("kind", [g.FAN_LINE_1D, g.FLAT], ids=lambda k: g.KIND_NAMES[k])
+def test_filter_widest_detector(torch_dev, kind):
+    """2730 detector columns, the widest row a 4096-point FFT filters without
+    wrap-around (2730 + 1365 = 4095); one more column is refused."""
     torch = torch_dev
-    wl = W.small_case(kind)
+    wl = W.small_case(kind, nu=2730, views=8, nv=12)
     proj = wl.projections()
     plan = g.Plan(wl.scan, wl.img)
+    assert plan.info().fft_len == 4096
+    p = torch.from_numpy(proj).cuda()
+    plan.filter(p)
+    out = torch.empty(wl.scan.proj_shape, dtype=torch.float32, device="cuda")
+    plan.store_filtered(out)
+    torch.cuda.synchronize()
+    cosw, red = O.weights(wl.scan)
+    weighted = (proj * cosw) * red[:, None, :]
+    t32, _ = O.ramp_taps(False, wl.scan.nu, wl.scan.du)
+    want = O.filter_rows(t32, weighted)
+    rep = _report(f"filter {g.KIND_NAMES[kind]} 2730", out.cpu().numpy(), want)
+    assert rep["max_rel_to_peak"] <= 1e-5
+    wide = W.small_case(kind, nu=2731, views=8, nv=12)
+    with pytest.raises(g.GctError) as e:
+        g.Plan(wide.scan, wide.img)
+    assert "2730 detector columns" in str(e.value)
+
+
+@pytest.mark.parametrize("nu", [96, 85, 170, 171])
+@pytest.mark.parametrize("kind", [g.PARALLEL_1D, g.FAN_ARC_1D, g.CONE_PARALLEL, g.FLAT], ids=lambda k: g.KIND_NAMES[k])
+def test_filter_stage_parity(torch_dev, kind, nu):
+    """Fused cosine + redundancy weighting + FFT ramp filter against the oracle's
+    weights and exact truncated convolution (the oracle's FFT length is the
+    reference's nextPow2(2 N)).  The plan's FFT length is nextPow2(N + N // 2):
+    85 -> 128 (the reference's 256), 170 -> 256 (512), and 171 -> 256 with no
+    slack (171 + 85 = 256: one point shorter and the kept outputs would wrap)."""
+    torch = torch_dev
+    wl = W.small_case(kind, nu=nu)
+    proj = wl.projections()
+    plan = g.Plan(wl.scan, wl.img)
+    assert plan.info().fft_len == max(8, 1 << (nu + nu // 2 - 1).bit_length())
     p = torch.from_numpy(proj).cuda()
     plan.filter(p)
     out = torch.empty(wl.scan.proj_shape, dtype=torch.float32, device="cuda")
