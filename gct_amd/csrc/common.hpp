@@ -79,10 +79,10 @@ std::vector<uint32_t> spiral_windows(const gct_ct_geom &g, const gct_image_geom 
 std::vector<double> ramp_taps_f64(int kind, uint32_t n, float width, float cutoff, int window,
                                   std::vector<float> *taps32);
 uint32_t next_pow2(uint32_t n);
-// FFT length of the ramp filter for rows of n samples: the shortest power of
-// two (at least 8) whose circular convolution with the n-tap kernel, placed at
-// indices -floor(n/2) .. n-1-floor(n/2) mod L, wraps nothing into outputs
-// 0 .. n-1: L >= n + floor(n/2).  The reference uses nextPow2(2 n)
+// FFT length of the ramp filter for rows of n samples: the shortest length
+// 2^k (at least 8) or 3 x 2^k (at least 48) whose circular convolution with
+// the n-tap kernel, placed at indices -floor(n/2) .. n-1-floor(n/2) mod L,
+// wraps nothing into outputs 0 .. n-1: L >= n + floor(n/2).  The reference uses nextPow2(2 n)
 // (ramp_filtering_fft.cuh:36-48); the kept outputs are the same linear
 // convolution either way.
 uint32_t fft_len(uint32_t n);

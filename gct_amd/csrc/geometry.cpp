@@ -47,7 +47,10 @@ uint32_t next_pow2(uint32_t n)
 
 uint32_t fft_len(uint32_t n)
 {
-    return std::max(8u, next_pow2(n + n / 2u));
+    const uint32_t need = std::max(8u, n + n / 2u);
+    const uint32_t p2 = next_pow2(need);
+    const uint32_t p3 = 3u * next_pow2((need + 2u) / 3u);   // 3 x 2^k >= need
+    return (p3 >= 48u && p3 < p2) ? p3 : p2;
 }
 
 RotationInfo rotation_info(const gct_ct_geom &g)

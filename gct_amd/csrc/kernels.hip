@@ -76,6 +76,18 @@ template <> __device__ inline void dft<2>(float2 *u)
     u[1] = csub(a, b);
 }
 
+// X1 = a + W b + W^2 d, X2 = a + W^2 b + W d with W = exp(-2 pi i / 3) = -1/2 - i sqrt(3)/2:
+// both are m -/+ i s (b - d) with m = a - (b + d) / 2, s = sqrt(3)/2.
+template <> __device__ inline void dft<3>(float2 *u)
+{
+    const float s = 0.86602540378443865f;
+    const float2 a = u[0], t = cadd(u[1], u[2]), w = csub(u[1], u[2]);
+    const float2 m = make_float2(fmaf(-0.5f, t.x, a.x), fmaf(-0.5f, t.y, a.y));
+    u[0] = cadd(a, t);
+    u[1] = make_float2(fmaf(s, w.y, m.x), fmaf(-s, w.x, m.y));
+    u[2] = make_float2(fmaf(-s, w.y, m.x), fmaf(s, w.x, m.y));
+}
+
 template <> __device__ inline void dft<4>(float2 *u)
 {
     const float2 a = cadd(u[0], u[2]), b = csub(u[0], u[2]);
@@ -182,11 +194,18 @@ __device__ inline void stockham_io(const float2 *tw, uint32_t tid, Ld ld, St st)
     if constexpr (SYNC_END) __syncthreads();
 }
 
-// Radix order: 16 while at least 16 remain (8 then 4 for 32), then the rest.
+// Radix order: 16 while at least 16 remain (8 then 4 for 32), then the rest;
+// for lengths 3 * 2^k the powers of two come first and the radix-3 pass last,
+// so every pass before it has a power-of-two product P (k = i mod P by mask).
 template <uint32_t rem>
 constexpr uint32_t next_radix()
 {
-    return rem == 32u ? 8u : (rem >= 16u ? 16u : rem);
+    if constexpr (rem % 3u == 0u) {
+        constexpr uint32_t p2 = rem / 3u;
+        return p2 >= 16u ? 16u : (p2 > 1u ? p2 : 3u);
+    } else {
+        return rem == 32u ? 8u : (rem >= 16u ? 16u : rem);
+    }
 }
 
 // The radix of the last pass of an L-point FFT.
@@ -247,12 +266,17 @@ __device__ inline float cos_weight(const FiltRec &f, float du, float dv, uint32_
 }
 
 // Threads per row pair: L below 64, one wavefront up to L = 1024, then L / 16
-// (one radix-16 butterfly per thread and pass); a 256-thread workgroup holds
-// 256 / NT row pairs, which share the LDS twiddle table.
+// (one radix-16 butterfly per thread and pass); for L = 3 * 2^k, the power of
+// two from 64 that covers the L / 16 butterflies of a radix-16 pass.  A
+// 256-thread workgroup holds 256 / NT row pairs, which share the LDS twiddle
+// table.
 template <uint32_t L>
 constexpr uint32_t filt_nt()
 {
-    return L < 64u ? L : (L <= 1024u ? 64u : L / 16u);
+    if constexpr (L % 3u == 0u)
+        return L <= 1024u ? 64u : (L <= 2048u ? 128u : 256u);
+    else
+        return L < 64u ? L : (L <= 1024u ? 64u : L / 16u);
 }
 constexpr uint32_t kFiltBlock = 256;
 
@@ -662,6 +686,13 @@ void launch_filter_L(const FilterLaunch &a, hipStream_t s)
         hipLaunchKernelGGL(kern, dim3((pairs + g - 1u) / g), dim3(kFiltBlock), 0, s, a);
     };
     switch (a.L) {
+    case 48: go(filter_kernel<KIND, S, 48>, filt_nt<48>()); break;
+    case 96: go(filter_kernel<KIND, S, 96>, filt_nt<96>()); break;
+    case 192: go(filter_kernel<KIND, S, 192>, filt_nt<192>()); break;
+    case 384: go(filter_kernel<KIND, S, 384>, filt_nt<384>()); break;
+    case 768: go(filter_kernel<KIND, S, 768>, filt_nt<768>()); break;
+    case 1536: go(filter_kernel<KIND, S, 1536>, filt_nt<1536>()); break;
+    case 3072: go(filter_kernel<KIND, S, 3072>, filt_nt<3072>()); break;
     case 8: go(filter_kernel<KIND, S, 8>, filt_nt<8>()); break;
     case 16: go(filter_kernel<KIND, S, 16>, filt_nt<16>()); break;
     case 32: go(filter_kernel<KIND, S, 32>, filt_nt<32>()); break;
@@ -672,7 +703,7 @@ void launch_filter_L(const FilterLaunch &a, hipStream_t s)
     case 1024: go(filter_kernel<KIND, S, 1024>, filt_nt<1024>()); break;
     case 2048: go(filter_kernel<KIND, S, 2048>, filt_nt<2048>()); break;
     case 4096: go(filter_kernel<KIND, S, 4096>, filt_nt<4096>()); break;
-    default: fail(GCT_ERR_UNSUPPORTED, "FFT length outside 8..4096");
+    default: fail(GCT_ERR_UNSUPPORTED, "FFT length not a power of two in 8..4096 or 3 x one in 48..3072");
     }
 }
 

@@ -37,6 +37,33 @@ namespace {
 
 // ---- host FFT (FP64) for the filter spectrum ------------------------------
 
+void fft64(std::vector<std::complex<double>> &a);
+
+// FFT of length 2^k or 3 x 2^k in FP64: three interleaved length-n/3
+// transforms joined by one radix-3 stage, X_k = sum_r W_n^(r k) S_r[k mod n/3].
+void fft64_any(std::vector<std::complex<double>> &a)
+{
+    const size_t n = a.size();
+    if ((n & (n - 1)) == 0) {
+        fft64(a);
+        return;
+    }
+    const size_t m = n / 3;
+    std::vector<std::complex<double>> s[3];
+    for (size_t r = 0; r < 3; ++r) {
+        s[r].resize(m);
+        for (size_t i = 0; i < m; ++i) s[r][i] = a[3 * i + r];
+        fft64(s[r]);
+    }
+    for (size_t k = 0; k < n; ++k) {
+        std::complex<double> acc = s[0][k % m];
+        for (size_t r = 1; r < 3; ++r)
+            acc += s[r][k % m] * std::polar(1.0, -2.0 * std::numbers::pi * static_cast<double>((r * k) % n) /
+                                                     static_cast<double>(n));
+        a[k] = acc;
+    }
+}
+
 void fft64(std::vector<std::complex<double>> &a)
 {
     const size_t n = a.size();
@@ -382,7 +409,7 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
         const int m = static_cast<int>(k) - half;
         h[static_cast<uint32_t>((m + static_cast<int>(p->L)) % static_cast<int>(p->L))] = taps32[k];
     }
-    fft64(h);
+    fft64_any(h);
     std::vector<float2> H(p->L), tw(p->L);
     for (uint32_t k = 0; k < p->L; ++k) {
         H[k] = make_float2(static_cast<float>(h[k].real() / p->L), static_cast<float>(h[k].imag() / p->L));

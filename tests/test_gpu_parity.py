@@ -99,19 +99,30 @@ def test_filter_widest_detector(torch_dev, kind):
     assert "2730 detector columns" in str(e.value)
 
 
-@pytest.mark.parametrize("nu", [96, 85, 170, 171])
+def _fft_len(n):
+    """The shortest 2^k >= 8 or 3 x 2^k >= 48 at least n + n // 2."""
+    need = max(8, n + n // 2)
+    p2 = 1 << (need - 1).bit_length()
+    p3 = 3 * (1 << ((need + 2) // 3 - 1).bit_length())
+    return p3 if 48 <= p3 < p2 else p2
+
+
+@pytest.mark.parametrize("nu", [96, 85, 170, 171, 32, 512])
 @pytest.mark.parametrize("kind", [g.PARALLEL_1D, g.FAN_ARC_1D, g.CONE_PARALLEL, g.FLAT], ids=lambda k: g.KIND_NAMES[k])
 def test_filter_stage_parity(torch_dev, kind, nu):
     """Fused cosine + redundancy weighting + FFT ramp filter against the oracle's
     weights and exact truncated convolution (the oracle's FFT length is the
-    reference's nextPow2(2 N)).  The plan's FFT length is nextPow2(N + N // 2):
-    85 -> 128 (the reference's 256), 170 -> 256 (512), and 171 -> 256 with no
-    slack (171 + 85 = 256: one point shorter and the kept outputs would wrap)."""
+    reference's nextPow2(2 N)).  The plan's FFT length is the shortest 2^k or
+    3 x 2^k of at least N + N // 2: 96 -> 192 (radices 16, 4, 3; the
+    reference's 256), 85 -> 128 (256), 170 -> 256 (512), and with no slack --
+    one point shorter and the kept outputs would wrap -- 171 -> 256,
+    32 -> 48 (16, 3) and 512 -> 768 (16, 16, 3)."""
     torch = torch_dev
     wl = W.small_case(kind, nu=nu)
     proj = wl.projections()
     plan = g.Plan(wl.scan, wl.img)
-    assert plan.info().fft_len == max(8, 1 << (nu + nu // 2 - 1).bit_length())
+    assert plan.info().fft_len == _fft_len(nu)
+    assert {96: 192, 85: 128, 170: 256, 171: 256, 32: 48, 512: 768}[nu] == _fft_len(nu)
     p = torch.from_numpy(proj).cuda()
     plan.filter(p)
     out = torch.empty(wl.scan.proj_shape, dtype=torch.float32, device="cuda")
