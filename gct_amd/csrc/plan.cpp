@@ -301,6 +301,79 @@ bool rows_inside_detector(const gct_plan &p, const std::vector<BpRec64> &rec64)
     return true;
 }
 
+// The LDS shape a plan without shape options starts from (the autotuner can
+// still replace it): the first of the shapes, in the order they rank on the
+// flat-cone and C-arm workloads, whose staging area holds the boxes of at
+// least 99 % of the (tile, view) pairs -- a box that does not fit sends its
+// view through HBM.  The boxes follow the kernel's rule (the tile's corners,
+// a row and column of margin, cut at the padded detector), here in FP64 from
+// the FP64 records, over a sample of tiles (every 8th along each axis, and the
+// last) and of views (24, evenly spaced).
+int default_lds_shape(const gct_plan &p, const std::vector<BpRec64> &rec64)
+{
+    const gct_ct_geom &g = p.geom;
+    const uint32_t nx = p.img.x.num_samples, ny = p.img.y.num_samples, nz = p.img.z.num_samples;
+    const uint32_t nbx = (nx + 15u) / 16u, nby = (ny + 15u) / 16u;
+    auto samples = [](uint32_t n) {
+        std::vector<uint32_t> s;
+        const uint32_t step = std::max(1u, n / 8u);
+        for (uint32_t i = 0; i < n; i += step) s.push_back(i);
+        if (s.back() != n - 1u) s.push_back(n - 1u);
+        return s;
+    };
+    std::vector<uint32_t> views;
+    for (uint32_t i = 0; i < 24u && i < p.V; ++i) views.push_back(uint32_t(uint64_t(i) * p.V / std::min(24u, p.V)));
+    // padded (column, row) of voxel (x, y, z) in view v, as setup_column computes them
+    auto project = [&](const BpRec64 &r, double x, double y, double z, double &col, double &row) {
+        auto f = [&](const double q[4]) { return q[0] + q[1] * x + q[2] * y + q[3] * z; };
+        if (g.kind == GCT_FLAT) {
+            const double nw = f(r.w);
+            col = f(r.u) / nw + r.k[0];
+            row = f(r.v) / nw + r.k[1];
+        } else if (g.kind == GCT_CYLINDRICAL) {
+            const double nu = f(r.u), nw = f(r.w);
+            col = std::atan2(nu, nw) * double(p.inv_dgamma) + r.k[0];
+            row = r.k[2] * f(r.v) / std::sqrt(nu * nu + nw * nw) + r.k[1];
+        } else {
+            col = r.u[0] + r.u[1] * x + r.u[2] * y;
+            row = r.v[0] + r.v[3] * z;
+        }
+    };
+    static constexpr int kOrder[] = {6, 5, 4, 3, 1};
+    for (int si : kOrder) {
+        const LdsShape &sh = kLdsShapes[si];
+        const uint32_t nbz = (nz + sh.zu - 1u) / sh.zu;
+        uint64_t total = 0, fit = 0;
+        for (uint32_t v : views)
+            for (uint32_t bz : samples(nbz))
+                for (uint32_t by : samples(nby))
+                    for (uint32_t bx : samples(nbx)) {
+                        double cmin = 1e300, cmax = -1e300, rmin = 1e300, rmax = -1e300;
+                        for (int c = 0; c < 8; ++c) {
+                            const double x = std::min(bx * 16u + ((c & 1) ? 15u : 0u), nx - 1u);
+                            const double y = std::min(by * 16u + ((c & 2) ? 15u : 0u), ny - 1u);
+                            const double z = std::min(bz * sh.zu + ((c & 4) ? sh.zu - 1u : 0u), nz - 1u);
+                            double col, row;
+                            project(rec64[v], x, y, z, col, row);
+                            cmin = std::min(cmin, col);
+                            cmax = std::max(cmax, col);
+                            rmin = std::min(rmin, row);
+                            rmax = std::max(rmax, row);
+                        }
+                        ++total;
+                        if (!(cmin > -1e6 && cmax < 1e6 && rmin > -1e6 && rmax < 1e6)) continue;
+                        const int64_t clo = std::max<int64_t>(int64_t(std::floor(cmin)) - 2, -1);
+                        const int64_t chi = std::min<int64_t>(int64_t(std::floor(cmax)) + 1, int64_t(p.nu) + 1);
+                        const int64_t rlo = std::max<int64_t>(int64_t(std::floor(rmin)) - 2, -1);
+                        const int64_t rhi = std::min<int64_t>(int64_t(std::floor(rmax)) + 1, int64_t(p.nv) + 1);
+                        const int64_t nc = chi - clo + 1, nr = rhi - rlo + 1;
+                        if (nc >= 2 && nr >= 2 && nc - 1 <= int64_t(sh.pairs) && nr <= int64_t(sh.rows)) ++fit;
+                    }
+        if (total > 0 && fit * 100u >= total * 99u) return si;
+    }
+    return 0;
+}
+
 // The arc/cylindrical mapping uses rho = |(r.eu, r.ew)|, which equals the
 // reference's |r| sqrt(1 - (r^.ev)^2) only for an orthonormal detector basis.
 void check_basis(const gct_plan &p)
@@ -393,6 +466,10 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
     std::vector<FiltRec> frec;
     build_records(*p, rec, rec64, frec);
     p->rows_inside = rows_inside_detector(*p, rec64);
+    const bool shape_free = !o.z_unroll && !o.view_block && !o.lds_rows && !o.lds_pairs;
+    const int auto_shape = (shape_free && variant == GCT_BP_LDS && p->circular && !is_1d(geom.kind))
+                               ? default_lds_shape(*p, rec64)
+                               : -1;
     upload(*p, p->d_brec, rec.data(), rec.size() * sizeof(BpRec));
     upload(*p, p->d_brec64, rec64.data(), rec64.size() * sizeof(BpRec64));
     upload(*p, p->d_frec, frec.data(), frec.size() * sizeof(FiltRec));
@@ -466,6 +543,13 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
     p->view_block = o.view_block;
     p->lds_rows = o.lds_rows;
     p->lds_pairs = o.lds_pairs;
+    if (auto_shape >= 0) {
+        const LdsShape &sh = kLdsShapes[auto_shape];
+        p->z_unroll = sh.zu;
+        p->view_block = sh.vb;
+        p->lds_rows = sh.rows;
+        p->lds_pairs = sh.pairs;
+    }
     if (!o.z_unroll && o.bp_kernel == GCT_BP_AUTO) {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find({geom.kind, img.x.num_samples, img.y.num_samples, img.dim == 3 ? img.z.num_samples : 1u,

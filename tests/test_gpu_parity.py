@@ -280,11 +280,28 @@ def test_lds_and_direct_kernels_identical_512(torch_dev):
             assert torch.equal(v, ref), f"kernel {bk} z_unroll {zu}: {(v != ref).sum().item()} voxels differ"
 
 
+def test_default_lds_shape(torch_dev):
+    """A plan with no shape option takes the first shape, in the order they
+    rank, whose staging area holds 99 % of its sampled boxes: on the flat-cone
+    workload (boxes up to 48 rows x 32 pairs for 32-slice tiles)
+    (32, 3, 48, 34), on the C-arm one (34 x 22) (32, 4, 36, 28) -- the
+    autotuner's picks for both -- and on a cylindrical slab, whose tiles near
+    the source span up to 67 pairs, the general (16, 2, 40, 40)."""
+    for wl, want in ((W.flat_cone_256(noise=False), (32, 3, 48, 34)), (W.carm_short_scan_512(), (32, 4, 36, 28)),
+                     (W.cylindrical_slab(0), (16, 2, 40, 40))):
+        plan = g.Plan(wl.scan, wl.img)
+        i = plan.info()
+        assert (i.z_unroll, i.view_block, i.lds_rows, i.lds_pairs) == want, wl.name
+        plan.close()
+
+
 def test_lds_shape_options(torch_dev):
     """Shape fields left at zero take the first matching shape; a combination
-    that is no shape is rejected with an invalid-argument error."""
+    that is no shape is rejected with an invalid-argument error.  With no
+    shape field set, the plan picks from the geometry (see
+    test_default_lds_shape)."""
     wl = W.small_case(g.FLAT)
-    cases = [(dict(), (16, 2, 40, 40)), (dict(z_unroll=32), (32, 3, 48, 34)),
+    cases = [(dict(z_unroll=16), (16, 2, 40, 40)), (dict(z_unroll=32), (32, 3, 48, 34)),
              (dict(lds_rows=32), (16, 2, 32, 40)), (dict(lds_pairs=48), (16, 2, 40, 48)),
              (dict(view_block=3), (16, 3, 28, 34)), (dict(z_unroll=32, view_block=4), (32, 4, 36, 28))]
     for kw, want in cases:
