@@ -465,10 +465,13 @@ __device__ inline void tile_boxes(const BpLaunch &a, uint32_t bx, uint32_t by, u
 // Staging one view's box into LDS as (value, right neighbour - value) pairs.
 // The wave's rows are rw, rw + RSTEP, ... (wave-uniform; with HALF, lanes
 // 32-63 take the row below), so row guards are scalar branches.  The loads
-// are raw buffer loads over the view's bytes: the row's byte offset is a
+// are raw buffer loads from the view's first byte: the row's byte offset is a
 // scalar operand and the lane's column offset a vector operand fixed for the
-// view, so a row costs no address arithmetic, and a read past the view's end
-// returns 0 instead of faulting.  Each lane loads one value and takes its right
+// view, so a row costs no address arithmetic.  Every load reads inside the
+// filtered buffer without relying on the buffer's range check: lanes past the
+// box's last column and the second half-row of a box's last row pair read at
+// most one padded row past the view, and the buffer ends with one spare zero
+// row for the last view.  Each lane loads one value and takes its right
 // neighbour from the next lane (a row of m pairs needs m + 1 values).  Lanes up
 // to TP store, whatever the box's width m: pairs at columns m .. TP - 1 of a
 // row are never read (the clamped column keeps both taps below column m), and
@@ -487,8 +490,8 @@ __device__ inline float stage_ld(__amdgpu_buffer_rsrc_t rs, uint32_t voff, uint3
     }
 }
 
-// Rows past the box are loaded from an offset past the view's end (0 back,
-// nothing fetched): no branch, and no value left undefined per row.
+// Rows past the box re-read the box's first row (a valid address; the value
+// is not stored): no branch, and no value left undefined per row.
 template <int S, int RSTEP, int NIT, bool HALF>
 __device__ inline void stage_load(__amdgpu_buffer_rsrc_t rs, uint32_t row0, uint32_t row_bytes, uint32_t voff,
                                   int rw, int nrows, float *v)
@@ -496,7 +499,7 @@ __device__ inline void stage_load(__amdgpu_buffer_rsrc_t rs, uint32_t row0, uint
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
         const int r = rw + it * RSTEP;
-        const uint32_t soff = r < nrows ? row0 + static_cast<uint32_t>(r) * row_bytes : 0x80000000u;
+        const uint32_t soff = r < nrows ? row0 + static_cast<uint32_t>(r) * row_bytes : row0;
         v[it] = stage_ld<S>(rs, voff, soff);
     }
 }
@@ -529,7 +532,8 @@ __device__ inline void stage_block(const typename Stor<S>::T *base, size_t view_
                                    uint32_t nvb, const Box *bx, float2 *tile, int wave, uint32_t lane)
 {
     using T = typename Stor<S>::T;
-    const uint32_t view_bytes = static_cast<uint32_t>(view_elems * sizeof(T));   // < 2^32 (checked by the plan)
+    // a view and the padded row after it (the next view's first, or the buffer's spare one)
+    const uint32_t view_bytes = static_cast<uint32_t>((view_elems + pitch) * sizeof(T));   // < 2^31 (plan check)
     const uint32_t row_bytes = pitch * static_cast<uint32_t>(sizeof(T));
     for (uint32_t j = 0; j < nvb; ++j) {
         const int ncols = __builtin_amdgcn_readfirstlane(bx[j].ncols);

@@ -416,9 +416,10 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
     validate_geom(geom);
     validate_image(geom, img);
     if (geom.det_row.num_samples > kMaxDetCols) fail(GCT_ERR_UNSUPPORTED, "more than 2730 detector columns");
-    // one padded filtered view (FP32: (Nv + 3) x pitch x 4 B) is addressed by 31-bit buffer offsets
+    // one padded filtered view and the row after it (FP32: (Nv + 4) x pitch x 4 B) are
+    // addressed by 31-bit buffer offsets
     if (!is_1d(geom.kind) &&
-        (uint64_t(geom.det_col.num_samples) + 3u) * ((kMaxDetCols + 3u + 15u) & ~15u) * 4u >= (uint64_t(1) << 31))
+        (uint64_t(geom.det_col.num_samples) + 4u) * ((kMaxDetCols + 3u + 15u) & ~15u) * 4u >= (uint64_t(1) << 31))
         fail(GCT_ERR_UNSUPPORTED, "more than 196000 detector rows");
     if (o.storage < GCT_STORE_F32 || o.storage > GCT_STORE_BF16) fail(GCT_ERR_INVALID_ARGUMENT, "unknown storage");
     if (o.bp_kernel < GCT_BP_AUTO || o.bp_kernel > GCT_BP_LDS) fail(GCT_ERR_INVALID_ARGUMENT, "unknown bp_kernel");
@@ -525,7 +526,9 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
         upload(*p, dbeta, beta.data(), beta.size() * sizeof(float));
         launch_redundancy(geom.kind, dfan.as<float>(), dbeta.as<float>(), p->nu, p->V, p->rot.range, mode, value,
                           p->d_red.as<float>(), p->stream);
-        const size_t fbytes = size_t(p->filt_views) * p->rows_pv * p->pitch * storage_bytes(o.storage);
+        // one spare zero row after the last view: the LDS staging reads up to one
+        // padded row past a view
+        const size_t fbytes = (size_t(p->filt_views) * p->rows_pv + 1u) * p->pitch * storage_bytes(o.storage);
         p->d_filt = DevBuf(fbytes, p->stream, p->ordered);
         hip_check(hipMemsetAsync(p->d_filt.p, 0, fbytes, p->stream), "hipMemsetAsync");   // zero padding stays zero
         // host vectors above must outlive their copies; dfan/dbeta free in order
@@ -673,7 +676,7 @@ uint64_t plan_device_bytes(const gct_ct_geom &g, const gct_image_geom &img, cons
     const uint64_t pitch = (nu + 3u + 15u) & ~uint64_t(15);
     const uint64_t V = g.num_proj;
     const uint64_t L = fft_len(uint32_t(nu));
-    uint64_t b = uint64_t(fviews ? fviews : V) * (nv + 3u) * pitch * storage_bytes(o.storage);
+    uint64_t b = (uint64_t(fviews ? fviews : V) * (nv + 3u) + 1u) * pitch * storage_bytes(o.storage);
     b += V * (sizeof(BpRec) + sizeof(BpRec64) + sizeof(FiltRec) + sizeof(gct_proj_geom)) + V * nu * 4u + 2u * L * 8u;
     (void)img;
     return b;
