@@ -28,8 +28,13 @@ served    The serving stream: 32 seeded requests of mixed geometry and size,
           server (4 streams, HBM-footprint admission), first saturated (all
           requests queued at once: capacity), then with Poisson arrivals at
           80 % of that capacity (latency).
+joseph    The Joseph forward projector alone (the served requests'
+          reprojection) on the largest request of the serving mix and on the
+          flat-cone workload's volume and scan: rays per second, device volume
+          and projections resident.
 
-    python bench.py [--steps K] [--warmup W] [--no-cpu-baseline] [--sections flat,perview,precision,batched,served]
+    python bench.py [--steps K] [--warmup W] [--no-cpu-baseline]
+                    [--sections flat,perview,precision,batched,served,joseph]
 """
 from __future__ import annotations
 
@@ -445,13 +450,46 @@ def run_served(a, g, W, torch, np):
     }
 
 
+def run_joseph(a, g, W, torch, np):
+    """Forward projection time of the largest served request (its volume does
+    not fit the caches: the projector re-reads it from beyond L2 view after
+    view) and of the flat-cone workload (it does)."""
+    reqs = [w for w in W.serving_mix() if not g.is_1d(w.scan.kind)]
+    big = max(reqs, key=lambda w: w.img.nvox * w.scan.num_proj)
+    res = {}
+    for wl in (big, W.flat_cone_256(noise=False)):
+        plan = g.Plan(wl.scan, wl.img)
+        vol = torch.empty(wl.img.shape, dtype=torch.float32, device="cuda")
+        g.phantom_rasterize_device(wl.img, wl.phantom, vol)
+        out = torch.empty(wl.scan.proj_shape, dtype=torch.float32, device="cuda")
+        stream = torch.cuda.current_stream()
+        plan.forward_project(vol, out, stream=stream)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(5):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            plan.forward_project(vol, out, stream=stream)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        ms = statistics.median(ts)
+        rays = wl.scan.num_proj * wl.scan.nv * wl.scan.nu
+        res[wl.name] = {"volume": list(wl.img.shape), "views": wl.scan.num_proj,
+                        "detector": [wl.scan.nv, wl.scan.nu], "rays": rays, "ms": round(ms, 3),
+                        "grays_per_s": round(rays / ms / 1e6, 3)}
+        plan.close()
+        del vol, out
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--sections", default="flat,perview,precision,batched,served")
+    ap.add_argument("--sections", default="flat,perview,precision,batched,served,joseph")
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy 5 times (reference kernel for HBM byte counters)")
@@ -493,6 +531,10 @@ def main():
     if "served" in sections:
         print("bench: served section", file=sys.stderr, flush=True)
         out["served"] = run_served(a, g, W, torch, np)
+        torch.cuda.empty_cache()
+    if "joseph" in sections:
+        print("bench: joseph section", file=sys.stderr, flush=True)
+        out["joseph"] = run_joseph(a, g, W, torch, np)
         torch.cuda.empty_cache()
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, proj, vol, a.threads)
