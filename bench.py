@@ -194,6 +194,8 @@ def run_flat(a, g, W, torch, np, sections):
             "bp_kernel": variant,
             "z_unroll": info.z_unroll,
             "lds_shape": {"view_block": info.view_block, "rows": info.lds_rows, "pairs": info.lds_pairs},
+            "tiles_round_robin": bool(info.round_robin),
+            "row_clamp": not info.rows_inside,
             "autotune_best_bp_ms": round(tuned_ms, 4),
             "fft_len": info.fft_len,
         },

@@ -793,7 +793,12 @@ void bp3d_lds_kernel(BpLaunch a)
     __shared__ float2 tile[VB * TR * TP];
     __shared__ Box boxes[2 * VB];
     const uint32_t nbx = (a.nx + 15u) / 16u, nby = (a.ny + 15u) / 16u;
-    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+    // Contiguous runs of tiles per XCD keep the detector region an XCD reads
+    // small (a cylindrical slab: 12.0 against 13.8 ms); dealt round robin, all
+    // XCDs read the same views' rows at once from the Infinity Cache they share
+    // and finish together (the flat cone: 3.05 against 3.19 ms).  The
+    // autotuner picks per scan shape.
+    const uint32_t t = a.round_robin ? blockIdx.x : xcd_tile(blockIdx.x, gridDim.x);
     const uint32_t bx = t % nbx, rest = t / nbx;
     bp_tile3d_lds<KIND, S, F64, ZU, TR, VB, TP, WIN, CLAMP>(a, bx, rest % nby, rest / nby, threadIdx.x, tile, boxes);
 }

@@ -156,6 +156,7 @@ struct BpLaunch {
     uint32_t lds_rows, lds_pairs; //   views per staged block, staged rows, pairs per staged row
     const uint32_t *zwin;         // spiral: [first, last) view window per z slice (device); null otherwise
     bool rows_inside;             // every voxel's row lies inside the detector: LDS kernel without row clamp
+    bool round_robin;             // LDS kernel: tiles dealt to the XCDs round robin, not in contiguous runs
 };
 
 // Shapes of the LDS backprojection kernel: voxels per thread along z, views

@@ -89,6 +89,7 @@ struct Tuning {
     uint32_t z_unroll;
     int variant;
     uint32_t view_block, lds_rows, lds_pairs;
+    bool round_robin;
 };
 std::mutex g_tune_mu;
 std::map<std::tuple<int, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, int, int, int>, Tuning> g_tune;
@@ -563,6 +564,7 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
             p->view_block = it->second.view_block;
             p->lds_rows = it->second.lds_rows;
             p->lds_pairs = it->second.lds_pairs;
+            p->round_robin = it->second.round_robin;
         }
     }
     return p;
@@ -630,6 +632,7 @@ BpLaunch bp_args(gct_plan &p, float *vol, uint32_t first, uint32_t n, bool accum
     a.lds_rows = p.lds_rows;
     a.lds_pairs = p.lds_pairs;
     a.rows_inside = p.rows_inside;
+    a.round_robin = p.round_robin;
     return a;
 }
 
@@ -783,6 +786,7 @@ extern "C" gct_status gct_plan_get_info(const gct_plan *p, gct_plan_info *info)
         info->lds_rows = lds ? kLdsShapes[shape].rows : 0u;
         info->lds_pairs = lds ? kLdsShapes[shape].pairs : 0u;
         info->rows_inside = p->rows_inside;
+        info->round_robin = lds && p->round_robin;
     });
 }
 
@@ -994,11 +998,13 @@ extern "C" gct_status gct_plan_autotune(gct_plan *p, float *vol, uint32_t *best_
         hipEvent_t e0, e1;
         hip_check(hipEventCreate(&e0), "event");
         hip_check(hipEventCreate(&e1), "event");
-        struct Cand { int variant; uint32_t zu, vb, rows, pairs; };
+        // every LDS shape with its tiles dealt to the XCDs in contiguous runs and round robin
+        struct Cand { int variant; uint32_t zu, vb, rows, pairs; bool rr; };
         std::vector<Cand> cands;
-        for (const LdsShape &sh : kLdsShapes) cands.push_back({GCT_BP_LDS, sh.zu, sh.vb, sh.rows, sh.pairs});
-        cands.push_back({GCT_BP_DIRECT, 8u, 0u, 0u, 0u});
-        cands.push_back({GCT_BP_DIRECT, 16u, 0u, 0u, 0u});
+        for (const bool rr : {false, true})
+            for (const LdsShape &sh : kLdsShapes) cands.push_back({GCT_BP_LDS, sh.zu, sh.vb, sh.rows, sh.pairs, rr});
+        cands.push_back({GCT_BP_DIRECT, 8u, 0u, 0u, 0u, false});
+        cands.push_back({GCT_BP_DIRECT, 16u, 0u, 0u, 0u, false});
         // Two interleaved rounds (the second only for candidates within 10 % of
         // the best so far); a candidate keeps the better of its medians of
         // five, so drift in clocks or caches does not decide between close ones.
@@ -1014,6 +1020,7 @@ extern "C" gct_status gct_plan_autotune(gct_plan *p, float *vol, uint32_t *best_
                 p->view_block = c.vb;
                 p->lds_rows = c.rows;
                 p->lds_pairs = c.pairs;
+                p->round_robin = c.rr;
                 const BpLaunch a = bp_args(*p, vol, 0, p->V, false, true, 0);
                 launch_backproject(a, s);   // warm-up
                 float t[5];
@@ -1035,6 +1042,7 @@ extern "C" gct_status gct_plan_autotune(gct_plan *p, float *vol, uint32_t *best_
         p->view_block = bc.vb;
         p->lds_rows = bc.rows;
         p->lds_pairs = bc.pairs;
+        p->round_robin = bc.rr;
         const uint32_t bz = bc.zu;
         const int bv = bc.variant;
         (void)hipEventDestroy(e0);
@@ -1043,7 +1051,7 @@ extern "C" gct_status gct_plan_autotune(gct_plan *p, float *vol, uint32_t *best_
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             g_tune[{p->geom.kind, p->img.x.num_samples, p->img.y.num_samples, p->img.dim == 3 ? p->img.z.num_samples : 1u,
-                    p->V, p->nu, p->nv, p->opts.storage, int(p->circular), int(p->setup_f64)}] = Tuning{bz, bv, bc.vb, bc.rows, bc.pairs};
+                    p->V, p->nu, p->nv, p->opts.storage, int(p->circular), int(p->setup_f64)}] = Tuning{bz, bv, bc.vb, bc.rows, bc.pairs, bc.rr};
         }
         if (best_zu) *best_zu = bz;
         if (best_ms) *best_ms = best;

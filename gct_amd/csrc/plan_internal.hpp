@@ -89,6 +89,7 @@ struct gct_plan {
     uint32_t z_unroll = 16;
     uint32_t view_block = 0;   // LDS kernel shape as requested (0: default of z_unroll; see kLdsShapes)
     uint32_t lds_rows = 0, lds_pairs = 0;
+    bool round_robin = false;   // LDS kernel: tiles dealt round robin to the XCDs (autotuned)
     int variant = GCT_BP_LDS;
     hipStream_t stream = nullptr;     // stream-ordered plans (serving) allocate and free on it
     bool ordered = false;

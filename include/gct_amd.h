@@ -249,6 +249,8 @@ typedef struct gct_plan_info {
     uint32_t lds_pairs;              /*   pairs per staged row */
     int32_t rows_inside;             /* 1 when every voxel projects inside the detector rows in every
                                         view: the LDS backprojection then runs without its row clamp */
+    int32_t round_robin;             /* 1 when the LDS backprojection deals its tiles to the XCDs round
+                                        robin instead of in contiguous runs (chosen by the autotuner) */
 } gct_plan_info;
 
 /* Builds per-view tables (FP64 on the host, stored FP32), the redundancy
