@@ -554,6 +554,12 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
         p->lds_rows = sh.rows;
         p->lds_pairs = sh.pairs;
     }
+    // Untuned LDS plans deal their tiles round robin unless the volume is at
+    // most 4 tiles deep: measured, round robin is 1.6 % faster over the 3-D
+    // requests of the serving mix (up to 6 % on flat cubes), contiguous runs
+    // 13 % faster on the 4-tile-deep cylindrical slab.
+    if (p->variant == GCT_BP_LDS && img.dim == 3)
+        p->round_robin = (img.z.num_samples + p->z_unroll - 1u) / p->z_unroll > 4u;
     if (!o.z_unroll && o.bp_kernel == GCT_BP_AUTO) {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune.find({geom.kind, img.x.num_samples, img.y.num_samples, img.dim == 3 ? img.z.num_samples : 1u,

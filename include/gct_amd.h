@@ -250,7 +250,8 @@ typedef struct gct_plan_info {
     int32_t rows_inside;             /* 1 when every voxel projects inside the detector rows in every
                                         view: the LDS backprojection then runs without its row clamp */
     int32_t round_robin;             /* 1 when the LDS backprojection deals its tiles to the XCDs round
-                                        robin instead of in contiguous runs (chosen by the autotuner) */
+                                        robin instead of in contiguous runs (the autotuner's choice, else
+                                        round robin for volumes more than 4 tiles deep) */
 } gct_plan_info;
 
 /* Builds per-view tables (FP64 on the host, stored FP32), the redundancy
