@@ -400,6 +400,27 @@ __device__ inline float next_lane(float x)
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xF, 0xF, true));
 }
 
+// Minimum and maximum over each aligned group of 8 lanes, by DPP moves (no LDS
+// round trip): quad_perm [1,0,3,2] and [2,3,0,1] reduce each quad, and
+// row_half_mirror (lane i <-> 7 - i) pairs the two quads of a group.
+template <int CTRL>
+__device__ inline float dpp_move(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+__device__ inline float min8(float x)
+{
+    x = fminf(x, dpp_move<0xB1>(x));
+    x = fminf(x, dpp_move<0x4E>(x));
+    return fminf(x, dpp_move<0x141>(x));
+}
+__device__ inline float max8(float x)
+{
+    x = fmaxf(x, dpp_move<0xB1>(x));
+    x = fmaxf(x, dpp_move<0x4E>(x));
+    return fmaxf(x, dpp_move<0x141>(x));
+}
+
 // A view's staged region.  Box row r, column c is padded pixel
 // (rlo + 1 + r, clo + 1 + c).  The backprojection clamps padded coordinates
 // to [c0, c1] and [r0, r1] (just below the last staged column and row, so
@@ -428,14 +449,8 @@ __device__ inline void tile_boxes(const BpLaunch &a, uint32_t bx, uint32_t by, u
     const float nuf = static_cast<float>(a.nu);
     const ColSetup s = setup_column<KIND, false, false>(a, vb0 + j, static_cast<float>(cx), static_cast<float>(cy), 0.0f, nuf);
     const float row = fmaf(static_cast<float>(cz), s.drow, s.row0);
-    float cmin = s.col, cmax = s.col, rmin = row, rmax = row;
-#pragma unroll
-    for (int off = 1; off < 8; off <<= 1) {
-        cmin = fminf(cmin, __shfl_xor(cmin, off));
-        cmax = fmaxf(cmax, __shfl_xor(cmax, off));
-        rmin = fminf(rmin, __shfl_xor(rmin, off));
-        rmax = fmaxf(rmax, __shfl_xor(rmax, off));
-    }
+    // the 8 corners of a view are lanes 8j .. 8j + 7, all active
+    const float cmin = min8(s.col), cmax = max8(s.col), rmin = min8(row), rmax = max8(row);
     if (corner != 0u) return;
     Box b{};
     const bool finite = cmin > -1.0e6f && cmax < 1.0e6f && rmin > -1.0e6f && rmax < 1.0e6f;
