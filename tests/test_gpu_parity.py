@@ -286,12 +286,17 @@ def test_default_lds_shape(torch_dev):
     workload (boxes up to 48 rows x 32 pairs for 32-slice tiles)
     (32, 3, 48, 34), on the C-arm one (34 x 22) (32, 4, 36, 28) -- the
     autotuner's picks for both -- and on a cylindrical slab, whose tiles near
-    the source span up to 67 pairs, the general (16, 2, 40, 40)."""
-    for wl, want in ((W.flat_cone_256(noise=False), (32, 3, 48, 34)), (W.carm_short_scan_512(), (32, 4, 36, 28)),
-                     (W.cylindrical_slab(0), (16, 2, 40, 40))):
+    the source span up to 67 pairs, the general (16, 2, 40, 40).  Untuned
+    plans deal their tiles round robin unless the volume is at most 4 tiles
+    deep."""
+    for wl, want, rr in ((W.flat_cone_256(noise=False), (32, 3, 48, 34), 1),
+                         (W.carm_short_scan_512(), (32, 4, 36, 28), 1),
+                         (W.cylindrical_slab(0), (16, 2, 40, 40), 0)):
         plan = g.Plan(wl.scan, wl.img)
         i = plan.info()
         assert (i.z_unroll, i.view_block, i.lds_rows, i.lds_pairs) == want, wl.name
+        # tiles dealt round robin unless the volume is at most 4 tiles deep (the 64-slice slab)
+        assert i.round_robin == rr, wl.name
         plan.close()
 
 
