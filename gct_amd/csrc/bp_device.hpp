@@ -132,7 +132,12 @@ __device__ inline ColSetup setup_column(const BpLaunch &a, uint32_t v, float fx,
                 nu = fma(r.u[3], W, nu);
                 nv = fma(r.v[3], W, nv);
             }
-            const double rd = 1.0 / nw;
+            // 1 / nw: v_rcp_f64 and two Newton steps (each squares the relative
+            // error) instead of the IEEE division sequence; the results are
+            // rounded to FP32 next
+            double rd = __builtin_amdgcn_rcp(nw);
+            rd = fma(fma(-nw, rd, 1.0), rd, rd);
+            rd = fma(fma(-nw, rd, 1.0), rd, rd);
             s.col = static_cast<float>(fma(nu, rd, r.k[0]));
             s.w = static_cast<float>(rd * rd);
             s.row0d = fma(nv, rd, r.k[1]);
