@@ -717,8 +717,9 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                     float drow = s.drow;
                     asm("" : "+v"(drow));
                     // G slices at a time: their LDS reads are issued before the first is
-                    // used, so a wave keeps several in flight
-                    constexpr int G = (WIN || KIND == GCT_CYLINDRICAL) ? 1 : (ZU >= 32 ? kLdsGroup : 2);
+                    // used, so a wave keeps several in flight (cylindrical tiles: pairs at
+                    // every z-unroll; their per-update weight takes registers of its own)
+                    constexpr int G = WIN ? 1 : (KIND == GCT_CYLINDRICAL || ZU < 32) ? 2 : kLdsGroup;
 #pragma unroll
                     for (int k0 = 0; k0 < ZU; k0 += G) {
                         float2 tp[G], bt[G];
