@@ -340,8 +340,8 @@ int default_lds_shape(const gct_plan &p, const std::vector<BpRec64> &rec64)
             row = r.v[0] + r.v[3] * z;
         }
     };
-    static constexpr int kOrder[] = {6, 5, 4, 3, 1};
-    for (int si : kOrder) {
+    // fraction of the sampled tile boxes that shape si stages whole
+    auto fit_fraction = [&](int si) {
         const LdsShape &sh = kLdsShapes[si];
         const uint32_t nbz = (nz + sh.zu - 1u) / sh.zu;
         uint64_t total = 0, fit = 0;
@@ -370,9 +370,15 @@ int default_lds_shape(const gct_plan &p, const std::vector<BpRec64> &rec64)
                         const int64_t nc = chi - clo + 1, nr = rhi - rlo + 1;
                         if (nc >= 2 && nr >= 2 && nc - 1 <= int64_t(sh.pairs) && nr <= int64_t(sh.rows)) ++fit;
                     }
-        if (total > 0 && fit * 100u >= total * 99u) return si;
-    }
-    return 0;
+        return total > 0 ? double(fit) / double(total) : 0.0;
+    };
+    static constexpr int kOrder[] = {6, 5, 4, 3, 1};
+    for (int si : kOrder)
+        if (fit_fraction(si) >= 0.99) return si;
+    // No shape stages 99 % of the boxes whole (wide boxes): shapes 1 and 0
+    // stage the same columns, and 1 fewer rows, so it is the faster of the
+    // two wherever rows are not what overflows.
+    return fit_fraction(1) >= fit_fraction(0) - 0.005 ? 1 : 0;
 }
 
 // The arc/cylindrical mapping uses rho = |(r.eu, r.ew)|, which equals the

@@ -146,7 +146,8 @@ typedef struct gct_recon_opts {
      * a zero field matches any value (the first matching shape is taken); all four zero = the
      * shape the autotuner found for this scan shape, else the first of (32, 4, 36, 28),
      * (32, 3, 48, 34), (16, 4, 24, 28), (16, 3, 28, 34), (16, 2, 32, 40) whose staging area holds
-     * 99 % of a sample of the scan's (tile, view) boxes, else (16, 2, 40, 40) */
+     * 99 % of a sample of the scan's (tile, view) boxes, else (16, 2, 32, 40) if it holds
+     * within 0.5 % as many of them as (16, 2, 40, 40), else (16, 2, 40, 40) */
     uint32_t view_block;             /* views staged per block */
     uint32_t lds_rows;               /* detector rows staged per view */
     uint32_t lds_pairs;              /* (value, difference) pairs staged per row */

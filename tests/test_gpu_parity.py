@@ -285,13 +285,14 @@ def test_default_lds_shape(torch_dev):
     rank, whose staging area holds 99 % of its sampled boxes: on the flat-cone
     workload (boxes up to 48 rows x 32 pairs for 32-slice tiles)
     (32, 3, 48, 34), on the C-arm one (34 x 22) (32, 4, 36, 28) -- the
-    autotuner's picks for both -- and on a cylindrical slab, whose tiles near
-    the source span up to 67 pairs, the general (16, 2, 40, 40).  Untuned
-    plans deal their tiles round robin unless the volume is at most 4 tiles
-    deep."""
+    autotuner's picks for both.  A cylindrical slab's tiles near the source
+    span up to 67 pairs, so no shape holds 99 %; (16, 2, 32, 40) then holds
+    as many of its boxes as (16, 2, 40, 40) (the columns overflow, not the
+    rows) and stages fewer rows, so it is the default.  Untuned plans deal
+    their tiles round robin unless the volume is at most 4 tiles deep."""
     for wl, want, rr in ((W.flat_cone_256(noise=False), (32, 3, 48, 34), 1),
                          (W.carm_short_scan_512(), (32, 4, 36, 28), 1),
-                         (W.cylindrical_slab(0), (16, 2, 40, 40), 0)):
+                         (W.cylindrical_slab(0), (16, 2, 32, 40), 0)):
         plan = g.Plan(wl.scan, wl.img)
         i = plan.info()
         assert (i.z_unroll, i.view_block, i.lds_rows, i.lds_pairs) == want, wl.name
