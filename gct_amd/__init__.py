@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 from pathlib import Path
 
 import numpy as np
@@ -199,15 +200,30 @@ def check(status: int) -> None:
         raise GctError(status, lib().gct_last_error().decode())
 
 
-def _ptr(a) -> int | None:
-    """Address of a numpy array, a torch tensor or an int."""
-    if a is None:
-        return None
-    if isinstance(a, int):
-        return a
-    if isinstance(a, np.ndarray):
-        return a.ctypes.data
+def _ptr(a) -> int:
+    """Address of a device tensor already checked by _dev."""
     return a.data_ptr()
+
+
+def _dev(t, shape, what: str) -> int:
+    """Address of a device buffer of float32 values of `shape`: a contiguous
+    float32 torch tensor on a GPU with exactly that many elements.  Anything
+    else -- a numpy array, a host tensor, an integer, a wrong size -- raises
+    before any library call: a kernel handed a host address or a short buffer
+    faults the GPU."""
+    torch = sys.modules.get("torch")
+    if torch is None or not isinstance(t, torch.Tensor):
+        raise TypeError(f"{what} must be a torch tensor on the GPU, not {type(t).__name__}")
+    if t.device.type != "cuda":
+        raise ValueError(f"{what} must be on the GPU, not on {t.device}")
+    if t.dtype != torch.float32:
+        raise ValueError(f"{what} must be float32, not {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{what} must be contiguous")
+    n = int(np.prod(shape))
+    if t.numel() != n:
+        raise ValueError(f"{what} holds {t.numel()} values; {tuple(shape)} needs {n}")
+    return t.data_ptr()
 
 
 # ---- geometry --------------------------------------------------------------
@@ -370,29 +386,51 @@ class Plan:
         return i
 
     def filter(self, proj, first: int = 0, n: int | None = None, stream=None):
+        """Weights and filters views [first, first + n) of the device stack
+        `proj` (all views, [V][Nv][Nu])."""
         n = self.scan.num_proj - first if n is None else n
-        check(lib().gct_plan_filter(self.h, _ptr(proj), first, n, _stream(stream)))
+        p = _dev(proj, self.scan.proj_shape, "proj")
+        check(lib().gct_plan_filter(self.h, p, first, n, _stream(stream)))
 
     def backproject(self, vol, first: int = 0, n: int | None = None, accumulate=False, finish=True, stream=None):
         n = self.scan.num_proj - first if n is None else n
-        check(lib().gct_plan_backproject(self.h, _ptr(vol), first, n, int(accumulate), int(finish), _stream(stream)))
+        v = _dev(vol, self.img.shape, "vol")
+        check(lib().gct_plan_backproject(self.h, v, first, n, int(accumulate), int(finish), _stream(stream)))
 
     def execute(self, proj, vol, stream=None):
-        check(lib().gct_plan_execute(self.h, _ptr(proj), _ptr(vol), _stream(stream)))
+        p, v = _dev(proj, self.scan.proj_shape, "proj"), _dev(vol, self.img.shape, "vol")
+        check(lib().gct_plan_execute(self.h, p, v, _stream(stream)))
 
     def load_filtered(self, filt, stream=None):
-        check(lib().gct_plan_load_filtered(self.h, _ptr(filt), _stream(stream)))
+        f = _dev(filt, self.scan.proj_shape, "filt")
+        check(lib().gct_plan_load_filtered(self.h, f, _stream(stream)))
 
     def store_filtered(self, filt, stream=None):
-        check(lib().gct_plan_store_filtered(self.h, _ptr(filt), _stream(stream)))
+        f = _dev(filt, self.scan.proj_shape, "filt")
+        check(lib().gct_plan_store_filtered(self.h, f, _stream(stream)))
 
     def forward_project(self, vol, proj, stream=None):
-        check(lib().gct_plan_forward_project(self.h, _ptr(vol), _ptr(proj), _stream(stream)))
+        v, p = _dev(vol, self.img.shape, "vol"), _dev(proj, self.scan.proj_shape, "proj")
+        check(lib().gct_plan_forward_project(self.h, v, p, _stream(stream)))
 
     def autotune(self, vol):
+        """Times the backprojection variants (vol is scratch: overwritten)."""
+        v = _dev(vol, self.img.shape, "vol")
         zu, ms = C.c_uint32(), C.c_float()
-        check(lib().gct_plan_autotune(self.h, _ptr(vol), C.byref(zu), C.byref(ms)))
+        check(lib().gct_plan_autotune(self.h, v, C.byref(zu), C.byref(ms)))
         return zu.value, ms.value
+
+
+def _batch_args(plans, projs, vols):
+    """Device addresses of one stack and one volume per plan, checked."""
+    if len(projs) != len(plans) or len(vols) != len(plans):
+        raise ValueError(f"{len(plans)} plans need as many projection stacks and volumes "
+                         f"(got {len(projs)} and {len(vols)})")
+    pa = (C.c_void_p * len(plans))(*[_dev(p, pl.scan.proj_shape, f"projs[{i}]")
+                                      for i, (p, pl) in enumerate(zip(projs, plans))])
+    va = (C.c_void_p * len(plans))(*[_dev(v, pl.img.shape, f"vols[{i}]")
+                                      for i, (v, pl) in enumerate(zip(vols, plans))])
+    return pa, va
 
 
 class Batch:
@@ -404,8 +442,7 @@ class Batch:
         self.h = h
 
     def execute(self, projs, vols, use_graph=True, stream=None):
-        pa = (C.c_void_p * len(projs))(*[_ptr(p) for p in projs])
-        va = (C.c_void_p * len(vols))(*[_ptr(v) for v in vols])
+        pa, va = _batch_args(self.plans, projs, vols)
         check(lib().gct_batch_execute(self.h, pa, va, int(use_graph), _stream(stream)))
 
     def close(self):
@@ -418,6 +455,7 @@ class Batch:
             self.close()
         except Exception:
             pass
+
 
 
 def _stream(s):
@@ -460,7 +498,7 @@ def phantom_project_device(scan: Scan, params: np.ndarray, out, i0: float = 0.0,
         raise ValueError(f"out must be a contiguous float32 tensor of shape {scan.proj_shape}")
     dim = 2 if is_1d(scan.kind) else 3
     check(lib().gct_phantom_project_device(C.byref(scan.c()), dim, params.ctypes.data, params.shape[0], i0,
-                                           noise_seed, _ptr(out), _stream(stream)))
+                                           noise_seed, _dev(out, scan.proj_shape, "out"), _stream(stream)))
     return out
 
 
@@ -468,8 +506,8 @@ def phantom_rasterize_device(img: Image, params: np.ndarray, out, stream=None):
     params = np.ascontiguousarray(params, np.float64)
     if tuple(out.shape) != tuple(img.shape) or out.dtype != _torch().float32 or not out.is_contiguous():
         raise ValueError(f"out must be a contiguous float32 tensor of shape {img.shape}")
-    check(lib().gct_phantom_rasterize_device(C.byref(img.c()), params.ctypes.data, params.shape[0], _ptr(out),
-                                             _stream(stream)))
+    check(lib().gct_phantom_rasterize_device(C.byref(img.c()), params.ctypes.data, params.shape[0],
+                                             _dev(out, img.shape, "out"), _stream(stream)))
     return out
 
 

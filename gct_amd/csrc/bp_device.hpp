@@ -265,7 +265,7 @@ constexpr uint32_t kZBlock = 16;
 // Slices whose LDS taps the LDS backprojection reads before using the first:
 // at z unroll 32 (4 resident waves, up to 128 VGPRs) four, at 16 (7 waves,
 // up to 72) two -- more spill.  Cylindrical kernels, whose per-update weight
-// holds more registers, read one at a time.
+// holds more registers, read pairs at every z unroll.
 constexpr int kLdsGroup = 4;
 
 // Whether the row axis of a view stays parallel to z: then u, w and the
@@ -512,11 +512,12 @@ __device__ inline void tile_boxes(const BpLaunch &a, uint32_t bx, uint32_t by, u
 // 32-63 take the row below), so row guards are scalar branches.  The loads
 // are raw buffer loads from the view's first byte: the row's byte offset is a
 // scalar operand and the lane's column offset a vector operand fixed for the
-// view, so a row costs no address arithmetic.  Every load reads inside the
-// filtered buffer without relying on the buffer's range check: lanes past the
-// box's last column and the second half-row of a box's last row pair read at
-// most one padded row past the view, and the buffer ends with one spare zero
-// row for the last view.  Each lane loads one value and takes its right
+// view, so a row costs no address arithmetic.  The buffer's range check covers
+// the lane's offset only, not the row's (a scalar operand), so every load must
+// fall inside the filtered buffer by itself: lanes past the box's last column
+// and the second half-row of a box's last row pair read at most pitch + 30
+// elements past the view, which the plan's spare elements after the last view
+// cover (filt_spare_elems).  Each lane loads one value and takes its right
 // neighbour from the next lane (a row of m pairs needs m + 1 values).  Lanes up
 // to TP store, whatever the box's width m: pairs at columns m .. TP - 1 of a
 // row are never read (the clamped column keeps both taps below column m), and
@@ -917,11 +918,13 @@ constexpr int kBatchTP = static_cast<int>(kLdsShapes[1].pairs);
 static_assert(kLdsShapes[1].zu == kBatchLdsZU, "batched LDS tiles use a z-unroll-16 shape");
 
 // Persistent backprojection over the scans of one group (same geometry kind,
-// storage and tile type; a batch launches one such kernel per group).  Work
-// items are (scan, tile) pairs numbered scan by scan; each workgroup takes
-// items from the group's device counter until none are left, so the grid
-// drains on its own whatever the mix of scan sizes.
-template <int KIND, int S, bool LDS>
+// storage, column set-up precision and tile type; a batch launches one such
+// kernel per group).  Work items are (scan, tile) pairs numbered scan by scan;
+// each workgroup takes items from the group's device counter until none are
+// left, so the grid drains on its own whatever the mix of scan sizes.  F64:
+// the FP64 per-column set-up of the scans' own plans (cone scans of 512^3 and
+// more), so that batched volumes equal the plans' own bit for bit.
+template <int KIND, int S, bool LDS, bool F64 = false>
 __global__ __launch_bounds__(256)
 __attribute__((amdgpu_waves_per_eu(LDS ? lds_waves_per_simd<static_cast<int>(kBatchLdsZU), kBatchTR, kBatchVB, kBatchTP>()
                                        : 1, 8)))
@@ -954,10 +957,10 @@ void bp_batch_kernel(const BatchScan *__restrict__ scans, uint32_t nscans,
             const uint32_t bx = local % sc.tiles_x, rest = local / sc.tiles_x;
             const uint32_t by = rest % sc.tiles_y, bz = rest / sc.tiles_y;
             if constexpr (LDS)
-                bp_tile3d_lds<KIND, S, false, static_cast<int>(kBatchLdsZU), kBatchTR, kBatchVB, kBatchTP>(
+                bp_tile3d_lds<KIND, S, F64, static_cast<int>(kBatchLdsZU), kBatchTR, kBatchVB, kBatchTP>(
                     a, bx, by, bz, tid, tile, boxes);
             else
-                bp_tile3d<KIND, S, false, false, kBatchZU>(a, bx, by, bz, tid);
+                bp_tile3d<KIND, S, false, F64, kBatchZU>(a, bx, by, bz, tid);
         }
     }
 }
@@ -1060,9 +1063,15 @@ void launch_batch_kind(const BatchScan *scans, const BatchGroup &g, uint32_t *co
         hipLaunchKernelGGL((bp_batch_kernel<KIND, S, false>), dim3(grid), dim3(256), 0, s, scans, g.num_scans,
                            g.total_tiles, counter);
     } else if constexpr (KIND == GCT_FLAT || KIND == GCT_CYLINDRICAL) {
-        if (g.lds)
+        if (g.lds && g.f64)
+            hipLaunchKernelGGL((bp_batch_kernel<KIND, S, true, true>), dim3(grid), dim3(256), 0, s, scans,
+                               g.num_scans, g.total_tiles, counter);
+        else if (g.lds)
             hipLaunchKernelGGL((bp_batch_kernel<KIND, S, true>), dim3(grid), dim3(256), 0, s, scans, g.num_scans,
                                g.total_tiles, counter);
+        else if (g.f64)
+            hipLaunchKernelGGL((bp_batch_kernel<KIND, S, false, true>), dim3(grid), dim3(256), 0, s, scans,
+                               g.num_scans, g.total_tiles, counter);
         else
             hipLaunchKernelGGL((bp_batch_kernel<KIND, S, false>), dim3(grid), dim3(256), 0, s, scans, g.num_scans,
                                g.total_tiles, counter);

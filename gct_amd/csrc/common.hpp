@@ -250,6 +250,7 @@ struct BatchScan {
 struct BatchGroup {
     int kind, storage;
     bool lds;
+    bool f64;                     // FP64 per-column set-up (the plans' own choice, kept so results match)
     uint32_t first_scan, num_scans, total_tiles;
 };
 void launch_backproject_batch_group(const BatchScan *scans_dev, const BatchGroup &g, uint32_t *counter, hipStream_t s);

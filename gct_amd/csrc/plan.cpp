@@ -376,8 +376,8 @@ int default_lds_shape(const gct_plan &p, const std::vector<BpRec64> &rec64)
     for (int si : kOrder)
         if (fit_fraction(si) >= 0.99) return si;
     // No shape stages 99 % of the boxes whole (wide boxes): shapes 1 and 0
-    // stage the same columns, and 1 fewer rows, so it is the faster of the
-    // two wherever rows are not what overflows.
+    // stage the same 40 pairs per row, and shape 1 8 fewer rows (32 against
+    // 40), so it is the faster of the two wherever rows are not what overflows.
     return fit_fraction(1) >= fit_fraction(0) - 0.005 ? 1 : 0;
 }
 
@@ -533,9 +533,9 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
         upload(*p, dbeta, beta.data(), beta.size() * sizeof(float));
         launch_redundancy(geom.kind, dfan.as<float>(), dbeta.as<float>(), p->nu, p->V, p->rot.range, mode, value,
                           p->d_red.as<float>(), p->stream);
-        // one spare zero row after the last view: the LDS staging reads up to one
-        // padded row past a view
-        const size_t fbytes = (size_t(p->filt_views) * p->rows_pv + 1u) * p->pitch * storage_bytes(o.storage);
+        // spare zero elements after the last view for the LDS staging loads
+        const size_t fbytes = (size_t(p->filt_views) * p->rows_pv * p->pitch + filt_spare_elems(p->pitch)) *
+                              storage_bytes(o.storage);
         p->d_filt = DevBuf(fbytes, p->stream, p->ordered);
         hip_check(hipMemsetAsync(p->d_filt.p, 0, fbytes, p->stream), "hipMemsetAsync");   // zero padding stays zero
         // host vectors above must outlive their copies; dfan/dbeta free in order
@@ -691,7 +691,8 @@ uint64_t plan_device_bytes(const gct_ct_geom &g, const gct_image_geom &img, cons
     const uint64_t pitch = (nu + 3u + 15u) & ~uint64_t(15);
     const uint64_t V = g.num_proj;
     const uint64_t L = fft_len(uint32_t(nu));
-    uint64_t b = (uint64_t(fviews ? fviews : V) * (nv + 3u) + 1u) * pitch * storage_bytes(o.storage);
+    uint64_t b = (uint64_t(fviews ? fviews : V) * (nv + 3u) * pitch + filt_spare_elems(uint32_t(pitch))) *
+                 storage_bytes(o.storage);
     b += V * (sizeof(BpRec) + sizeof(BpRec64) + sizeof(FiltRec) + sizeof(gct_proj_geom)) + V * nu * 4u + 2u * L * 8u;
     (void)img;
     return b;
@@ -1005,11 +1006,36 @@ extern "C" gct_status gct_plan_autotune(gct_plan *p, float *vol, uint32_t *best_
         if (!p || !vol) fail(GCT_ERR_INVALID_ARGUMENT, "null argument");
         if (p->filt_views < p->V) fail(GCT_ERR_INVALID_ARGUMENT, "plan holds a partial filtered buffer");
         DeviceGuard dg(p->device);
-        hipStream_t s = nullptr;
-        hip_check(hipStreamCreate(&s), "hipStreamCreate");
-        hipEvent_t e0, e1;
-        hip_check(hipEventCreate(&e0), "event");
-        hip_check(hipEventCreate(&e1), "event");
+        // The stream and events are released, and the plan keeps its previous
+        // shape, whatever a timing launch throws.
+        struct Res {
+            hipStream_t s = nullptr;
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            ~Res()
+            {
+                if (s) (void)hipStreamSynchronize(s);
+                if (e0) (void)hipEventDestroy(e0);
+                if (e1) (void)hipEventDestroy(e1);
+                if (s) (void)hipStreamDestroy(s);
+            }
+        } res;
+        struct Restore {
+            gct_plan &p;
+            uint32_t zu, vb, rows, pairs;
+            int variant;
+            bool rr, armed = true;
+            ~Restore()
+            {
+                if (!armed) return;
+                p.z_unroll = zu; p.view_block = vb; p.lds_rows = rows; p.lds_pairs = pairs;
+                p.variant = variant; p.round_robin = rr;
+            }
+        } restore{*p, p->z_unroll, p->view_block, p->lds_rows, p->lds_pairs, p->variant, p->round_robin};
+        hip_check(hipStreamCreate(&res.s), "hipStreamCreate");
+        hip_check(hipEventCreate(&res.e0), "event");
+        hip_check(hipEventCreate(&res.e1), "event");
+        hipStream_t s = res.s;
+        hipEvent_t e0 = res.e0, e1 = res.e1;
         // every LDS shape with its tiles dealt to the XCDs in contiguous runs and round robin
         struct Cand { int variant; uint32_t zu, vb, rows, pairs; bool rr; };
         std::vector<Cand> cands;
@@ -1049,6 +1075,7 @@ extern "C" gct_status gct_plan_autotune(gct_plan *p, float *vol, uint32_t *best_
             }
         }
         const Cand bc = cands[bi];
+        restore.armed = false;
         p->variant = bc.variant;
         p->z_unroll = bc.zu;
         p->view_block = bc.vb;
@@ -1057,9 +1084,6 @@ extern "C" gct_status gct_plan_autotune(gct_plan *p, float *vol, uint32_t *best_
         p->round_robin = bc.rr;
         const uint32_t bz = bc.zu;
         const int bv = bc.variant;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        (void)hipStreamDestroy(s);
         {
             std::lock_guard<std::mutex> lk(g_tune_mu);
             g_tune[{p->geom.kind, p->img.x.num_samples, p->img.y.num_samples, p->img.dim == 3 ? p->img.z.num_samples : 1u,
@@ -1070,200 +1094,3 @@ extern "C" gct_status gct_plan_autotune(gct_plan *p, float *vol, uint32_t *best_
     });
 }
 
-// ---- batches ------------------------------------------------------------------
-
-struct gct_batch {
-    static constexpr uint32_t kSideStreams = 4;
-    int device = 0;
-    std::vector<gct_plan *> plans;    // creation order
-    std::vector<uint32_t> order;      // work order: by kernel group, largest scans first
-    std::vector<BatchScan> scans;     // in work order
-    std::vector<BatchGroup> groups;   // contiguous ranges of `scans`
-    DevBuf d_scans, d_counter;
-    // Groups run as concurrent branches: forked from the launching stream to
-    // side streams by events and joined back the same way (also under capture).
-    hipStream_t side[kSideStreams] = {};
-    hipEvent_t fork = nullptr, join[kSideStreams] = {};
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    std::vector<const float *> g_proj;
-    std::vector<float *> g_vol;
-    hipStream_t g_stream = nullptr;
-    ~gct_batch()
-    {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        for (uint32_t i = 0; i < kSideStreams; ++i) {
-            if (side[i]) (void)hipStreamDestroy(side[i]);
-            if (join[i]) (void)hipEventDestroy(join[i]);
-        }
-        if (fork) (void)hipEventDestroy(fork);
-    }
-};
-
-extern "C" gct_status gct_batch_create(gct_plan *const *plans, uint32_t n, gct_batch **out)
-{
-    return guarded([&] {
-        if (!plans || !n || !out) fail(GCT_ERR_INVALID_ARGUMENT, "empty batch");
-        auto b = std::make_unique<gct_batch>();
-        b->device = plans[0] ? plans[0]->device : 0;
-        for (uint32_t i = 0; i < n; ++i) {
-            if (!plans[i]) fail(GCT_ERR_INVALID_ARGUMENT, "null plan");
-            if (plans[i]->device != b->device) fail(GCT_ERR_INVALID_ARGUMENT, "plans on different devices");
-            if (plans[i]->filt_views < plans[i]->V) fail(GCT_ERR_INVALID_ARGUMENT, "plan holds a partial filtered buffer");
-            if (plans[i]->setup_f64) fail(GCT_ERR_UNSUPPORTED, "batched backprojection uses the FP32 setup");
-            b->plans.push_back(plans[i]);
-        }
-        b->order.resize(n);
-        for (uint32_t i = 0; i < n; ++i) b->order[i] = i;
-        auto cost = [&](uint32_t i) {
-            const gct_plan *p = b->plans[i];
-            return uint64_t(p->img.x.num_samples) * p->img.y.num_samples * (p->img.dim == 3 ? p->img.z.num_samples : 1u) * p->V;
-        };
-        auto key = [&](uint32_t i) {
-            const gct_plan *p = b->plans[i];
-            return std::make_tuple(p->geom.kind, int(p->opts.storage), batch_tile_depth(p->geom.kind, p->circular));
-        };
-        std::stable_sort(b->order.begin(), b->order.end(), [&](uint32_t a, uint32_t c) {
-            return key(a) != key(c) ? key(a) < key(c) : cost(a) > cost(c);
-        });
-        for (uint32_t k = 0; k < n; ++k) {
-            const gct_plan &p = *b->plans[b->order[k]];
-            if (k == 0 || key(b->order[k]) != key(b->order[k - 1])) {
-                const uint32_t depth = batch_tile_depth(p.geom.kind, p.circular);
-                b->groups.push_back(BatchGroup{p.geom.kind, int(p.opts.storage),
-                                               !is_1d(p.geom.kind) && depth == kBatchLdsZU, k, 0, 0});
-            }
-            ++b->groups.back().num_scans;
-        }
-        DeviceGuard dg(b->device);
-        b->d_counter = DevBuf(b->groups.size() * sizeof(uint32_t));
-        hip_check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
-        for (uint32_t i = 0; i < gct_batch::kSideStreams; ++i) {
-            hip_check(hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking), "hipStreamCreate");
-            hip_check(hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming), "hipEventCreate");
-        }
-        *out = b.release();
-    });
-}
-
-extern "C" void gct_batch_destroy(gct_batch *b)
-{
-    if (!b) return;
-    int prev = 0;
-    (void)hipGetDevice(&prev);
-    (void)hipSetDevice(b->device);
-    delete b;
-    (void)hipSetDevice(prev);
-}
-
-namespace gct {
-namespace {
-
-// Builds and uploads the work table (synchronous; never inside a capture).
-void batch_prepare(gct_batch &b, const float *const *proj, float *const *vol)
-{
-    (void)proj;
-    b.scans.clear();
-    uint32_t tile = 0;
-    size_t gi = 0;
-    for (uint32_t k = 0; k < b.order.size(); ++k) {
-        const uint32_t oi = b.order[k];
-        if (gi < b.groups.size() && b.groups[gi].first_scan == k) tile = 0;   // counts restart per group
-        gct_plan &p = *b.plans[oi];
-        if (!vol[oi]) fail(GCT_ERR_INVALID_ARGUMENT, "null volume");
-        BatchScan sc{};
-        sc.bp = bp_args(p, vol[oi], 0, p.V, false, true, 0);
-        const uint32_t depth = batch_tile_depth(p.geom.kind, sc.bp.circular);
-        sc.bp.z_unroll = depth;
-        sc.bp.variant = depth == kBatchLdsZU ? GCT_BP_LDS : GCT_BP_DIRECT;
-        sc.tiles_x = (p.img.x.num_samples + 15u) / 16u;
-        sc.tiles_y = (p.img.y.num_samples + 15u) / 16u;
-        sc.tiles_z = is_1d(p.geom.kind) ? 1u : (p.img.z.num_samples + depth - 1u) / depth;
-        sc.tile_begin = tile;
-        tile += sc.tiles_x * sc.tiles_y * sc.tiles_z;
-        b.scans.push_back(sc);
-        BatchGroup &g = b.groups[gi];
-        g.total_tiles = tile;
-        if (k + 1 == g.first_scan + g.num_scans) ++gi;
-    }
-    const size_t bytes = b.scans.size() * sizeof(BatchScan);
-    if (b.d_scans.bytes != bytes) b.d_scans = DevBuf(bytes);
-    hip_check(hipMemcpy(b.d_scans.p, b.scans.data(), bytes, hipMemcpyHostToDevice), "upload batch table");
-}
-
-// Stream work only: filters of every scan, then one persistent backprojection.
-void batch_record(gct_batch &b, const float *const *proj, hipStream_t s)
-{
-    for (uint32_t oi : b.order) {
-        gct_plan &p = *b.plans[oi];
-        if (!proj[oi]) fail(GCT_ERR_INVALID_ARGUMENT, "null projections");
-        launch_filter(filter_args(p, proj[oi], 0, 0, p.V, 0), s);
-    }
-    uint32_t *counters = b.d_counter.as<uint32_t>();
-    launch_zero_counters(counters, uint32_t(b.groups.size()), s);
-    if (b.groups.size() == 1) {
-        launch_backproject_batch_group(b.d_scans.as<BatchScan>(), b.groups[0], counters, s);
-        return;
-    }
-    hip_check(hipEventRecord(b.fork, s), "fork");
-    const uint32_t used = std::min<uint32_t>(gct_batch::kSideStreams, uint32_t(b.groups.size()));
-    for (uint32_t i = 0; i < used; ++i) hip_check(hipStreamWaitEvent(b.side[i], b.fork, 0), "fork wait");
-    for (size_t gi = 0; gi < b.groups.size(); ++gi)
-        launch_backproject_batch_group(b.d_scans.as<BatchScan>(), b.groups[gi], counters + gi,
-                                       b.side[gi % gct_batch::kSideStreams]);
-    for (uint32_t i = 0; i < used; ++i) {
-        hip_check(hipEventRecord(b.join[i], b.side[i]), "join");
-        hip_check(hipStreamWaitEvent(s, b.join[i], 0), "join wait");
-    }
-}
-
-}  // namespace
-}  // namespace gct
-
-extern "C" gct_status gct_batch_execute(gct_batch *b, const float *const *proj, float *const *vol, int use_graph,
-                                        void *stream)
-{
-    return guarded([&] {
-        if (!b || !proj || !vol) fail(GCT_ERR_INVALID_ARGUMENT, "null argument");
-        DeviceGuard dg(b->device);
-        hipStream_t s = static_cast<hipStream_t>(stream);
-        const size_t n = b->plans.size();
-        std::vector<const float *> P(proj, proj + n);
-        std::vector<float *> Vv(vol, vol + n);
-        if (!use_graph) {
-            batch_prepare(*b, proj, vol);
-            batch_record(*b, proj, s);
-            return;
-        }
-        const bool same = b->exec && b->g_stream == s && b->g_proj == P && b->g_vol == Vv;
-        if (!same) {
-            if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
-            if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
-            hip_check(hipStreamSynchronize(s), "sync before capture");
-            batch_prepare(*b, proj, vol);
-            hipStream_t cap = nullptr;
-            hip_check(hipStreamCreate(&cap), "hipStreamCreate");
-            hip_check(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "begin capture");
-            try {
-                batch_record(*b, proj, cap);
-            } catch (...) {
-                hipGraph_t g = nullptr;
-                (void)hipStreamEndCapture(cap, &g);
-                if (g) (void)hipGraphDestroy(g);
-                (void)hipStreamDestroy(cap);
-                throw;
-            }
-            hipGraph_t g = nullptr;
-            const hipError_t e = hipStreamEndCapture(cap, &g);
-            (void)hipStreamDestroy(cap);
-            hip_check(e, "end capture");
-            b->graph = g;
-            hip_check(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph instantiate");
-            b->g_proj = P;
-            b->g_vol = Vv;
-            b->g_stream = s;
-        }
-        hip_check(hipGraphLaunch(b->exec, s), "graph launch");
-    });
-}

@@ -115,4 +115,36 @@ uint64_t volume_bytes(const gct_image_geom &img);
 size_t storage_bytes(int storage);
 gct_recon_opts opts_or_default(const gct_recon_opts *o);
 
+// Elements of a plan's filtered buffer beyond its last view.  The LDS staging
+// loads of a view start at a padded row r <= nv + 2 (the view's last) and
+// column c <= nu + 1, and add a lane offset of at most 63 elements, or one
+// row and 31 elements in the half-row form; with pitch >= nu + 3 they reach at
+// most pitch + 30 elements past the view's end (62 in the full-row form).  The
+// buffer's range check does not cover the row offset (a scalar operand), so
+// the buffer itself must: two spare zero rows and 64 elements do, with a row
+// to spare.
+inline uint64_t filt_spare_elems(uint32_t pitch) { return 2u * uint64_t(pitch) + 64u; }
+
+// ---- batches (batch.cpp) -----------------------------------------------------
+// Allocation-free pieces shared by gct_batch_* and gct_batch_execute_noalloc.
+
+// Rejects an empty list, null plans, plans on several devices and plans with a
+// partial filtered buffer.
+void check_batch_plans(gct_plan *const *plans, uint32_t n);
+// Work order of the scans: grouped by (kind, storage, FP64 set-up, tile type),
+// larger scans first within a group, creation order among equals.
+void batch_order(gct_plan *const *plans, uint32_t n, uint32_t *order);
+// Groups of consecutive scans in `order` that share a persistent kernel; returns
+// how many (GCT_ERR_UNSUPPORTED beyond max_groups).
+uint32_t batch_groups(gct_plan *const *plans, const uint32_t *order, uint32_t n, BatchGroup *groups,
+                      uint32_t max_groups);
+// One scan's work-table entry, its tiles numbered from tile_begin.
+BatchScan batch_entry(gct_plan &p, float *vol, uint32_t tile_begin);
+// The work table in work order (tile numbers restart per group; sets the
+// groups' total_tiles).
+void batch_table(gct_plan *const *plans, const uint32_t *order, uint32_t n, float *const *vol, BatchGroup *groups,
+                 uint32_t ngroups, BatchScan *table);
+// Fused weighting + filter launches of every scan, in work order.
+void batch_filters(gct_plan *const *plans, const uint32_t *order, uint32_t n, const float *const *proj, hipStream_t s);
+
 }  // namespace gct

@@ -292,11 +292,17 @@ gct_status gct_plan_forward_project(gct_plan *plan, const float *vol_dev, float 
 
 typedef struct gct_batch gct_batch;
 
-/* plans must stay alive while the batch exists; they are not copied. */
+/* plans must stay alive while the batch exists; they are not copied.  Plans of
+ * any kind, storage and set-up precision mix: scans that share a specialised
+ * kernel (kind, storage, FP64 set-up, tile type) form a group. */
 gct_status gct_batch_create(gct_plan *const *plans, uint32_t num_plans, gct_batch **out);
 void gct_batch_destroy(gct_batch *batch);
-/* Filters every scan, then backprojects all of them in one persistent launch.
- * use_graph != 0 captures the sequence once into a hipGraph and replays it. */
+/* Filters every scan, then backprojects each group in one persistent launch
+ * (groups run side by side).  use_graph != 0 captures the sequence once into
+ * a hipGraph and replays it while the stream and buffers stay the same; the
+ * graph and the plain path keep separate device work tables.  Volumes equal
+ * each plan's own gct_plan_execute bit for bit.  A batch, like its plans,
+ * must not execute on two streams at once. */
 gct_status gct_batch_execute(gct_batch *batch, const float *const *proj_dev, float *const *vol_dev,
                              int use_graph, void *stream);
 
@@ -306,7 +312,8 @@ gct_status gct_batch_execute(gct_batch *batch, const float *const *proj_dev, flo
  * LDS-staged with z-unroll 16 (view blocks 2, 3, 4) or 32 (view blocks 1, 2),
  * direct with z-unroll 8 or 16 -- and stores the fastest in the plan (and a
  * process-wide cache keyed by shape; gct_plan_get_info reports the choice).
- * Uses vol_dev as scratch. */
+ * Uses vol_dev as scratch: its contents are overwritten.  On an error the plan
+ * keeps the kernel shape it had. */
 gct_status gct_plan_autotune(gct_plan *plan, float *vol_dev, uint32_t *best_z_unroll, float *best_ms);
 
 /* ---- serving: continuous batching of reconstruction requests ------------ */

@@ -93,6 +93,78 @@ def test_batch_graph_equals_individual_plans(torch_dev):
         assert np.array_equal(v.cpu().numpy(), ref)
 
 
+def test_batch_graph_and_plain_paths_keep_their_own_tables(torch_dev):
+    """Graph replay into volumes A, a plain execute into volumes B, then a
+    replay into A again: the replay writes A (its table still names A's
+    buffers), and B keeps the plain execute's result."""
+    torch = torch_dev
+    wls = [W.small_case(g.FLAT, n=32, views=60), W.small_case(g.FAN_ARC_1D, n=40, views=72),
+           W.small_case(g.CYLINDRICAL, n=32, views=48)]
+    projs = [torch.from_numpy(w.projections()).cuda() for w in wls]
+    plans = [g.Plan(w.scan, w.img) for w in wls]
+    want = []
+    for pl, p, w in zip(plans, projs, wls):
+        v = torch.empty(w.img.shape, dtype=torch.float32, device="cuda")
+        pl.execute(p, v)
+        want.append(v)
+    batch = g.Batch(plans)
+    A = [torch.zeros(w.img.shape, dtype=torch.float32, device="cuda") for w in wls]
+    B = [torch.zeros(w.img.shape, dtype=torch.float32, device="cuda") for w in wls]
+    batch.execute(projs, A, use_graph=True)
+    batch.execute(projs, B, use_graph=False)
+    torch.cuda.synchronize()
+    for a, b, ref in zip(A, B, want):
+        assert torch.equal(a, ref) and torch.equal(b, ref)
+        a.fill_(-1.0)
+        b.fill_(-7.0)
+    batch.execute(projs, A, use_graph=True)
+    torch.cuda.synchronize()
+    for a, b, ref in zip(A, B, want):
+        assert torch.equal(a, ref)
+        assert bool((b == -7.0).all()), "the replay wrote into the plain execute's volumes"
+    batch.close()
+
+
+def test_batch_with_fp64_setup_scan(torch_dev):
+    """A 512^3 cone scan (FP64 per-column set-up) batched with small FP32
+    scans: every volume equals its own plan's bit for bit (the 512^3 plan's
+    parity with the oracle is tested on the same workload in
+    test_gpu_precision.py)."""
+    torch = torch_dev
+    carm = W.carm_short_scan_512()
+    wls = [carm, W.small_case(g.FLAT, n=48, views=90), W.small_case(g.FAN_ARC_1D, n=64, views=90),
+           W.small_case(g.CYLINDRICAL, n=40, views=60)]
+    projs = [carm.projections_device()] + [torch.from_numpy(w.projections()).cuda() for w in wls[1:]]
+    plans = [g.Plan(w.scan, w.img) for w in wls]
+    assert plans[0].info().setup_f64 == 1 and plans[1].info().setup_f64 == 0
+    want = []
+    for pl, p, w in zip(plans, projs, wls):
+        v = torch.empty(w.img.shape, dtype=torch.float32, device="cuda")
+        pl.execute(p, v)
+        want.append(v)
+    batch = g.Batch(plans)
+    for use_graph in (True, False):
+        vols = [torch.full(w.img.shape, 9.0, dtype=torch.float32, device="cuda") for w in wls]
+        batch.execute(projs, vols, use_graph=use_graph)
+        torch.cuda.synchronize()
+        for w, v, ref in zip(wls, vols, want):
+            assert torch.equal(v, ref), f"{w.name} (graph {use_graph}): {(v != ref).sum().item()} voxels differ"
+        del vols
+    batch.close()
+
+
+@pytest.mark.parametrize("storage,size", [(g.STORE_F32, 4), (g.STORE_F16, 2)], ids=["f32", "f16"])
+def test_filtered_buffer_spare_elements(torch_dev, storage, size):
+    """The filtered buffer holds V views of (Nv + 3) padded rows and then
+    2 * pitch + 64 spare zero elements, which the LDS staging loads of the
+    last view may reach (tests/test_staging_bounds.py replays them)."""
+    wl = W.small_case(g.FLAT, n=32, views=40, nu=45, nv=6)
+    plan = g.Plan(wl.scan, wl.img, storage=storage)
+    i = plan.info()
+    assert i.filt_pitch == 48 and i.filt_rows_per_view == 9
+    assert i.filt_bytes == (40 * 9 * 48 + 2 * 48 + 64) * size
+
+
 def test_autotuner_picks_a_valid_variant_and_keeps_results(torch_dev):
     torch = torch_dev
     wl = W.small_case(g.FLAT, n=64, views=120)
