@@ -164,6 +164,7 @@ def _declare(L: C.CDLL) -> None:
         "gct_batch_create": (st, [C.POINTER(P), C.c_uint32, C.POINTER(P)]),
         "gct_batch_destroy": (None, [P]),
         "gct_batch_execute": (st, [P, C.POINTER(P), C.POINTER(P), C.c_int, P]),
+        "gct_batch_execute_noalloc": (st, [C.POINTER(P), C.c_uint32, C.POINTER(P), C.POINTER(P), P]),
         "gct_server_create": (st, [C.POINTER(ServerOpts), C.POINTER(P)]),
         "gct_server_destroy": (None, [P]),
         "gct_server_submit": (st, [P, C.POINTER(CTGeom), C.POINTER(ImageGeom), C.POINTER(ReconOpts), P, P, C.c_int,
@@ -456,6 +457,18 @@ class Batch:
         except Exception:
             pass
 
+
+NOALLOC_MAX_SCANS = 1024   # GCT_NOALLOC_MAX_SCANS
+
+
+def batch_execute_noalloc(plans, projs, vols, stream=None):
+    """gct_batch_execute_noalloc: the plans' scans backprojected by the
+    persistent batch kernels with the work table and counters in the
+    library's fixed device array -- no device allocation."""
+    plans = list(plans)
+    pa, va = _batch_args(plans, projs, vols)
+    arr = (C.c_void_p * len(plans))(*[p.h.value for p in plans])
+    check(lib().gct_batch_execute_noalloc(arr, len(plans), pa, va, _stream(stream)))
 
 
 def _stream(s):

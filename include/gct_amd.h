@@ -306,6 +306,24 @@ void gct_batch_destroy(gct_batch *batch);
 gct_status gct_batch_execute(gct_batch *batch, const float *const *proj_dev, float *const *vol_dev,
                              int use_graph, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc: gct_batch_create + gct_batch_execute(use_graph = 0) +
+ * gct_batch_destroy in one call that allocates nothing (no hipMalloc,
+ * hipFree, host staging or stream/event creation).  The work table and work
+ * counters live in a fixed array of the library on each device (144 KiB:
+ * 256 B of counters and GCT_NOALLOC_MAX_SCANS table entries), held until the
+ * process ends; the table reaches it through kernel arguments.  plans[i]
+ * reconstructs proj_dev[i] into vol_dev[i], as the plan's own execute would,
+ * bit for bit.  At most GCT_NOALLOC_MAX_SCANS scans and 64 kernel groups
+ * (GCT_ERR_INVALID_ARGUMENT / GCT_ERR_UNSUPPORTED beyond).  Every call on one
+ * device shares that device's array: calls that overlap on one device
+ * (several host threads, or streams that run concurrently) race and must be
+ * serialised by the caller.  Groups run one after another on `stream`. */
+#define GCT_NOALLOC_MAX_SCANS 1024
+gct_status gct_batch_execute_noalloc(gct_plan *const *plans, uint32_t num_plans, const float *const *proj_dev,
+                                     float *const *vol_dev, void *stream);
+
 /* ---- autotuner ----------------------------------------------------------- */
 
 /* Times backprojection for the plan's shape over the candidate kernels --
