@@ -122,11 +122,23 @@ _lib = None
 
 
 def lib() -> C.CDLL:
-    """The loaded library; raises if it has not been built."""
+    """The loaded library; raises if it has not been built.
+
+    The library binds the HIP runtime already loaded in the process.  PyTorch
+    for ROCm ships its own HIP and HSA runtimes; loaded after this library,
+    they would sit beside the system ones it pulled in, and two runtimes in
+    one process cannot share the GPU (the second finds no device).  So when
+    torch is installed it is imported first, and the library then runs on
+    torch's runtime: one runtime, whose device pointers and streams both
+    sides use."""
     global _lib
     if _lib is None:
         if not LIB_PATH.exists():
             raise ImportError(f"{LIB_PATH} is missing: build it with `python -m gct_amd.build`")
+        if "torch" not in sys.modules:
+            import importlib.util
+            if importlib.util.find_spec("torch") is not None:
+                import torch  # noqa: F401  (its HIP runtime must be the one the library binds)
         L = C.CDLL(str(LIB_PATH))
         _declare(L)
         _lib = L

@@ -915,6 +915,12 @@ extern "C" gct_status gct_reconstruct(const gct_ct_geom *geom, const float *proj
                 chunk = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(geom->num_proj, (free_b / 16) / per)));
             }
             chunk = std::min(chunk, geom->num_proj);
+            // The filter transforms detector rows in pairs (2k, 2k + 1) counted from the
+            // launch's first view, and a pair's rounding depends on both rows.  Every
+            // chunk must start at an even row, as the whole stack's launch does, so
+            // that both paths pair (and round) alike: with an odd Nv, an even number
+            // of views per chunk.
+            if ((nv & 1u) && (chunk & 1u) && chunk < geom->num_proj) chunk = chunk > 1u ? chunk - 1u : 2u;
         }
         auto plan = make_plan(*geom, *img, o, chunk);
         hipStream_t s = nullptr;

@@ -184,6 +184,44 @@ def spiral_case(kind: int = CYLINDRICAL, n: int = 48, rotations: float = 3.0, vi
     return Workload(f"spiral_{kind}", scan, img, ph, 0.0, seed)
 
 
+# The reference's end-to-end FDK regression geometries (test/cuda/recon/
+# fdk_axial_tests.cu:15-537): 360 views at 1 deg (the flat short scan at
+# 0.0105434 rad), 256 x 256 (x 5) images at 1 mm, the detector pitches and
+# source distances of each case.  The reference reads its projections and
+# reference volumes from data files that are not in its repository; the
+# detector widths here are chosen to cover the image (the reference's are
+# unknown) and the projections are analytic line integrals of seeded
+# ellipses / ellipsoids.  name: (kind, columns, column pitch, rows, row pitch,
+# SID, SDD, angular increment)
+REGRESSION_CASES = {
+    "1dpdct": (PARALLEL_1D, 410, 0.8961, 1, 1.0, 0.0, 0.0, None),
+    "1dadct": (FAN_ARC_1D, 420, 0.0017922, 1, 1.0, 500.0, 1000.0, None),
+    "1dldct": (FAN_LINE_1D, 290, 2.0, 1, 1.0, 1000.0, 1536.0, None),
+    "2dcdct": (CYLINDRICAL, 420, 0.0017922, 16, 1.14, 500.0, 1000.0, None),
+    "2dfdct": (FLAT, 290, 2.0, 12, 2.0, 1000.0, 1536.0, None),
+    "2dfdct_short": (FLAT, 290, 2.0, 12, 2.0, 1000.0, 1536.0, 0.0105434),
+    "2dpdct": (PARALLEL_2D, 410, 0.8961, 8, 1.14, 0.0, 0.0, None),
+    "2drdct": (CONE_PARALLEL, 410, 0.8961, 12, 1.14, 500.0, 1000.0, None),
+}
+
+
+def reference_regression(name: str, seed: int = 4242) -> Workload:
+    kind, nu, du, nv, dv, sid, sdd, incr = REGRESSION_CASES[name]
+    if incr is None:
+        incr = float(np.float32(1.0 * math.pi / 180.0))   # static_cast<float>(1.0 * pi / 180.0)
+    scan = circular_scan(kind, nu, du, 360, incr, nv=nv, dv=dv, det_row_center=(nu - 1) / 2.0,
+                         det_col_center=(nv - 1) / 2.0, src_det_dist=sdd, src_iso_dist=sid)
+    # densities in mm^-1, as for every 256 mm field of view here (water is ~0.02)
+    if is_1d(kind):
+        img = Image(256, 256, 1.0, 1.0)
+        ph = phantom_random(seed, 10, 2, 128.0, (0.05, 0.4), (-0.03, 0.1))
+    else:
+        img = Image(256, 256, 1.0, 1.0, 5, 1.0)
+        ph = phantom_random(seed, 16, 3, 128.0, (0.1, 0.45), (-0.02, 0.1))
+        ph[:, 2] *= 0.05   # centres near the 5-slice slab, so that most ellipsoids cross it
+    return Workload(f"regression_{name}", scan, img, ph, 0.0, seed)
+
+
 def small_case(kind: int, n: int = 64, views: int = 120, nu: int = 96, nv: int = 80, short_scan: bool = False,
                direction: int = 1, seed: int = 42) -> Workload:
     """Small instance of every geometry for parity tests against the oracle."""

@@ -265,7 +265,9 @@ gct_status gct_plan_get_info(const gct_plan *plan, gct_plan_info *info);
 
 /* Stage 1+2 (fused): cosine and redundancy weighting and ramp filtering of the
  * device projections into the plan's padded buffer (fdk_axial.cuh:170-178).
- * first_view/num_views select a contiguous range of views (all: 0, num_proj). */
+ * first_view/num_views select a contiguous range of views (all: 0, num_proj).
+ * Rows are transformed in pairs counted from first_view; a range filters to
+ * the same bits as the whole stack when first_view x Nv is even. */
 gct_status gct_plan_filter(gct_plan *plan, const float *proj_dev, uint32_t first_view,
                            uint32_t num_views, void *stream);
 

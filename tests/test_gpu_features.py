@@ -165,6 +165,25 @@ def test_filtered_buffer_spare_elements(torch_dev, storage, size):
     assert i.filt_bytes == (40 * 9 * 48 + 2 * 48 + 64) * size
 
 
+def test_plan_before_torch_touches_the_gpu():
+    """A fresh process that loads the library and creates a plan before it
+    imports torch: the Python layer imports torch before loading the library,
+    so both run on torch's HIP runtime (one runtime in the process) and torch
+    still finds the GPU afterwards."""
+    import subprocess
+    import sys
+    code = ("import gct_amd as g; from gct_amd import workloads as W\n"
+            "wl = W.small_case(g.FLAT, n=16, views=20, nu=24, nv=20)\n"
+            "plan = g.Plan(wl.scan, wl.img)\n"
+            "import torch\n"
+            "v = torch.zeros(wl.img.shape, device='cuda'); p = torch.zeros(wl.scan.proj_shape, device='cuda')\n"
+            "plan.execute(p, v); torch.cuda.synchronize(); print('ok', float(v.abs().sum()))\n")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=240,
+                       cwd=str(__import__("pathlib").Path(__file__).resolve().parent.parent))
+    print(r.stdout, r.stderr[-2000:])
+    assert r.returncode == 0 and r.stdout.startswith("ok 0.0")
+
+
 def test_autotuner_picks_a_valid_variant_and_keeps_results(torch_dev):
     torch = torch_dev
     wl = W.small_case(g.FLAT, n=64, views=120)
