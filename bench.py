@@ -12,8 +12,13 @@ flat      The headline number.  A step is one plain FDK reconstruction of the
           autotuned once before the warm-up.  `value` is voxel x view updates
           per second over the timed steps; the backprojection kernel's
           roofline is measured with HIP events on its stream.
+          Beside it: the same reconstruction with the transfers (pinned
+          host projections in, volume out), and the kernel's VALU issue
+          utilisation from its committed instruction counters.
 perview   The same backprojection launched one view at a time with a
           full-volume read-modify-write per view (the reference's structure).
+c1        The 2-D parallel-beam workload (256^2, 360 views, 367 bins): the
+          GPU plan and the oracle's FP32 restatement on the host cores.
 precision The short-scan C-arm workload (512^3, 248 views over 200 deg) with
           FP32, FP16 and BF16 filtered-projection storage.
 batched   The uneven batch: 64 fan-arc slices of seeded sizes plus 16
@@ -51,8 +56,50 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 F_ALG_FLAT = 45          # FP32 operations per voxel x view update of the flat-cone mapping + bilinear + accumulate
+# the same count for the other geometries (the reference's per-update formula)
+F_ALG = {"parallel_1d": 13, "fan_line_1d": 22, "fan_arc_1d": 26, "parallel_2d": 24, "cone_parallel": 25,
+         "cylindrical": 56, "flat": 45}
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
+N_CU, CLOCK_HZ = 256, 2.4e9   # MI355X: one wave64 VALU instruction per CU per cycle (4 SIMD16, non-packed)
+
+
+def host_cores():
+    """Threads the CPU legs use: the cores this process may run on (its
+    affinity mask), at most 16 -- a one-GPU box's share of its host."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return min(16, avail), {"os_cpu_count": os.cpu_count(), "affinity": avail}
+
+
+def valu_roof_gups(kind_name):
+    return PEAK_FP32_TFLOPS * 1e12 / F_ALG[kind_name] / 1e9
+
+
+def roofline_entry(kind_name, updates, bp_ms, alg_bytes):
+    """Backprojection kernel against the FP32-VALU roof of its geometry
+    (F_ALG operations per update) and the HBM roof of its compulsory bytes."""
+    s = bp_ms * 1e-3
+    gups = updates / s / 1e9
+    return {"bp_ms": round(bp_ms, 4), "bp_gups": round(gups, 2), "valu_roof_gups": round(valu_roof_gups(kind_name), 1),
+            "frac_of_valu_roof": round(gups / valu_roof_gups(kind_name), 4),
+            "alg_tflops": round(F_ALG[kind_name] * updates / s / 1e12, 2),
+            "alg_hbm_gbs": round(alg_bytes / s / 1e9, 1), "hbm_frac": round(alg_bytes / s / 1e9 / PEAK_HBM_GBS, 5)}
+
+
+def timed_events(torch, stream, fn, reps):
+    """Per-repetition milliseconds of fn() on `stream`, by HIP events on that stream."""
+    out = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        out.append(e0.elapsed_time(e1))
+    return out
 
 
 def _percentile(xs, q):
@@ -64,39 +111,82 @@ def _percentile(xs, q):
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
 
 
-def cpu_baseline(wl, proj, gpu_vol, nthreads):
-    """The CPU oracle (FP32 restatement of the reference path) timed on this
-    host: weighting + ramp filter of every view, backprojection of a sample of
-    z slices, scaled to the whole volume.  Also reports how the GPU volume
-    agrees with the oracle on those slices."""
+def cpu_baseline(wl, proj, gpu_vol, nthreads, cpu_info):
+    """The CPU oracle (FP32 restatement of the reference path; the reference
+    has no CPU FDK) timed on this host: weighting + ramp filter of every view
+    and backprojection of a 64-slice sample of the volume, scaled to the whole
+    volume.  One warm-up run, then the median of 3.  Also reports how the GPU
+    volume agrees with the oracle on the sampled slices."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as O
 
     scan, img = wl.scan, wl.img
-    nsample = 32
+    nsample = 64
     z0 = img.nz // 2 - nsample // 2
-    t = time.perf_counter()
-    cosw, red = O.weights(scan)
-    weighted = (proj * cosw) * red[:, None, :]
     t32, _ = O.ramp_taps(False, scan.nu, scan.du, 1.0, wl.window)
-    filt = O.filter_rows(t32, weighted, nthreads=nthreads)
-    t_filter = time.perf_counter() - t
-    t = time.perf_counter()
-    vol = O.backproject(scan, img, filt, z0=z0, z1=z0 + nsample, nthreads=nthreads)
-    t_bp = time.perf_counter() - t
-    full = t_filter + t_bp * img.nz / nsample
+    runs = []
+    for _ in range(4):
+        t = time.perf_counter()
+        cosw, red = O.weights(scan)
+        weighted = (proj * cosw) * red[:, None, :]
+        filt = O.filter_rows(t32, weighted, nthreads=nthreads)
+        t_filter = time.perf_counter() - t
+        del weighted, cosw
+        t = time.perf_counter()
+        vol = O.backproject(scan, img, filt, z0=z0, z1=z0 + nsample, nthreads=nthreads)
+        t_bp = time.perf_counter() - t
+        runs.append((t_filter + t_bp * img.nz / nsample, t_filter, t_bp))
+        del filt
+    runs = sorted(runs[1:])   # the first run is the warm-up
+    full, t_filter, t_bp = runs[1]
     rep = O.parity_report(gpu_vol[z0:z0 + nsample], vol[z0:z0 + nsample])
     return {
         "value": round(wl.updates / full / 1e9, 4),
         "unit": "GUPS",
         "cores": nthreads,
+        "host": cpu_info,
         "kind": "port",
-        "sample": (f"oracle FP32: weighting+filter of all {scan.num_proj} views ({t_filter:.2f} s) + "
-                   f"backprojection of z slices [{z0},{z0 + nsample}) of {img.nz} ({t_bp:.2f} s), "
-                   f"scaled to the full volume"),
+        "sample": (f"oracle FP32 on {nthreads} threads: weighting+filter of all {scan.num_proj} views "
+                   f"({t_filter:.2f} s) + backprojection of z slices [{z0},{z0 + nsample}) of {img.nz} "
+                   f"({t_bp:.2f} s), scaled to the full volume; median of 3 after a warm-up"),
         "bp_gups": round(nsample * img.nx * img.ny * scan.num_proj / t_bp / 1e9, 4),
+        "runs_s": [round(r[0], 3) for r in runs],
         "gpu_vs_oracle_on_sample": rep,
     }
+
+
+# ---- config 1: 2-D parallel-beam FDK (the reference's CPU-runnable case) -------
+
+def run_c1(a, g, W, torch, np, nthreads):
+    """256^2 image from 360 parallel views of 367 bins: the GPU plan (HIP
+    events, projections resident) and the oracle's FP32 restatement on the
+    host cores (median of 3 after a warm-up)."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O
+
+    wl = W.parallel_1d_256()
+    proj = wl.projections()
+    p = torch.from_numpy(proj).cuda()
+    v = torch.empty(wl.img.shape, dtype=torch.float32, device="cuda")
+    plan = g.Plan(wl.scan, wl.img)
+    stream = torch.cuda.current_stream()
+    for _ in range(3):
+        plan.execute(p, v, stream=stream)
+    ms = timed_events(torch, stream, lambda: plan.execute(p, v, stream=stream), 20)
+    gpu = v.cpu().numpy()
+    cpu_s = []
+    for _ in range(4):
+        t = time.perf_counter()
+        want = O.fdk(wl.scan, wl.img, proj, nthreads=nthreads)
+        cpu_s.append(time.perf_counter() - t)
+    cpu = statistics.median(cpu_s[1:])
+    gms = statistics.median(ms)
+    plan.close()
+    return {"workload": "parallel_1d_256: 256^2 @0.5 mm, 360 views/180 deg, 367 bins @0.5 mm, Ram-Lak",
+            "updates": wl.updates, "gpu_ms": round(gms, 4), "gpu_gups": round(wl.updates / gms / 1e6, 2),
+            "cpu_oracle_ms": round(cpu * 1e3, 2), "cpu_oracle_gups": round(wl.updates / cpu / 1e9, 4),
+            "cpu_threads": nthreads, "gpu_vs_cpu": round(cpu * 1e3 / gms, 1),
+            "gpu_vs_oracle": O.parity_report(gpu, want)}
 
 
 # ---- flat-cone plain FDK (the headline) -----------------------------------------
@@ -158,7 +248,7 @@ def run_flat(a, g, W, torch, np, sections):
     # the kernel the tuned plan launches, named as rocprofv3 names it; the
     # counter-measured traffic applies only when the profile is of that kernel
     kernel = _bp_kernel_name(g, info)
-    traffic, traffic_kernel = None, None
+    traffic, traffic_kernel, valu_insts = None, None, None
     tfile = ROOT / "profiles" / "bp_flat_cone_256_traffic.json"
     if tfile.exists():
         try:
@@ -166,8 +256,26 @@ def run_flat(a, g, W, torch, np, sections):
             traffic_kernel = tj.get("kernel")
             if traffic_kernel == kernel:
                 traffic = tj.get("hbm_bytes_per_launch")
+                valu_insts = tj.get("valu_insts_per_launch")
         except (OSError, ValueError):
             traffic = None
+    # VALU issue utilisation: the profiled kernel's wave instructions over the
+    # issue slots of this run's launch (one per CU per cycle at 2.4 GHz)
+    valu_issue = round(valu_insts / (N_CU * CLOCK_HZ * bp_mean * 1e-3), 4) if valu_insts else None
+
+    # End to end with the transfers: pinned host projections in, volume out
+    hp = torch.empty(wl.scan.proj_shape, dtype=torch.float32, pin_memory=True)
+    hp.copy_(torch.from_numpy(proj))
+    hv = torch.empty(wl.img.shape, dtype=torch.float32, pin_memory=True)
+
+    def e2e():
+        p.copy_(hp, non_blocking=True)
+        plan.execute(p, v, stream=stream)
+        hv.copy_(v, non_blocking=True)
+
+    e2e()
+    e2e_ms = statistics.median(timed_events(torch, stream, e2e, 5))
+    del hp, hv
 
     vol = v.cpu().numpy()
     truth = g.phantom_rasterize(wl.img, wl.phantom)
@@ -211,11 +319,30 @@ def run_flat(a, g, W, torch, np, sections):
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 4),
+            "frac_basis": "algorithmic FLOPs of the reference's per-update formula (45 per update) over the "
+                          "FP32 peak; the kernel evaluates them strength-reduced along z in fewer instructions, "
+                          "so its own limit is valu_issue_frac",
+            "valu_issue_frac": valu_issue,
+            "valu_insts_per_update": round(valu_insts / upd, 2) if valu_insts else None,
             "traffic": traffic,
             "traffic_profiled_kernel": traffic_kernel,
             "alg_flops_per_update": F_ALG_FLAT,
             "alg_hbm_bytes_per_launch": alg_bytes,
             "hbm_frac_of_peak_alg": round(alg_bytes / (bp_mean * 1e-3) / 1e9 / PEAK_HBM_GBS, 5),
+        },
+        "end_to_end_with_pcie": {
+            "what": "H2D of the projections from pinned memory + weighting/filter + backprojection + D2H of "
+                    "the volume, one stream",
+            "ms": round(e2e_ms, 3), "gups": round(upd / (e2e_ms * 1e-3) / 1e9, 1),
+            "bytes_moved": int(proj.nbytes + wl.img.nvox * 4),
+        },
+        "targets": {
+            "served_vs_plain_5x": "measured in 'served'; the plain path already launches one fused kernel per "
+                                  "scan at the VALU issue limit, so batching and serving can only recover "
+                                  "launch, set-up, transfer and tail time, not multiply kernel throughput",
+            "hbm_roofline_60pct": "unattainable for backprojection by construction: its compulsory bytes are "
+                                  "0.055 B/update, so at the VALU roof it moves ~2 % of HBM peak "
+                                  "(hbm_frac_of_peak_alg); the kernel is bounded by VALU issue instead",
         },
         "input_generation_s": round(t_gen, 2),
     }
@@ -274,12 +401,17 @@ def run_precision(a, g, W, torch, np):
             ts.append(e0.elapsed_time(e1))
         ms = statistics.median(ts)
         info = plan.info()
+        bp_ms = statistics.median(timed_events(torch, stream, lambda: plan.backproject(v, stream=stream), 3))
+        plan.execute(p, v, stream=stream)
+        sbytes = 4 if st == g.STORE_F32 else 2
+        alg_bytes = wl.scan.num_proj * wl.scan.nv * wl.scan.nu * sbytes + wl.img.nvox * 4
         d = v - truth
         f = {"ms": round(ms, 3), "gups": round(wl.updates / (ms * 1e-3) / 1e9, 2),
              "bp_variant": {"kernel": "lds" if info.bp_kernel == g.BP_LDS else "direct",
                             "z_unroll": int(info.z_unroll), "view_block": int(info.view_block),
                             "rows": int(info.lds_rows), "pairs": int(info.lds_pairs)},
              "filtered_bytes": int(info.filt_bytes), "setup_f64": bool(info.setup_f64),
+             "roofline": roofline_entry("flat", wl.updates, bp_ms, alg_bytes),
              "rmse_vs_phantom": float(torch.sqrt(torch.mean(d * d)))}
         if ref is None:
             ref = v.clone()
@@ -341,6 +473,25 @@ def run_batched(a, g, W, torch, np):
     t_async = timed(plain_async)
     t_batch = timed(lambda: batch.execute(projs, vols, use_graph=True, stream=stream))
     n_slices = sum(1 for w in wls if g.is_1d(w.scan.kind))
+    # Backprojection kernels alone (each plan's own launch, after its filter),
+    # per geometry: the cylindrical slabs (all one shape: the first one) and
+    # the sum over the fan-arc slices.
+    roof = {}
+    slab_i = next(i for i, w in enumerate(wls) if not g.is_1d(w.scan.kind))
+    pl, p, v, w = plans[slab_i], projs[slab_i], vols[slab_i], wls[slab_i]
+    pl.execute(p, v, stream=stream)
+    slab_ms = statistics.median(timed_events(torch, stream, lambda: pl.backproject(v, stream=stream), 3))
+    roof["cylindrical_slab"] = roofline_entry("cylindrical", w.updates, slab_ms,
+                                              w.scan.num_proj * w.scan.nv * w.scan.nu * 4 + w.img.nvox * 4)
+    arc_ms, arc_upd, arc_bytes = 0.0, 0, 0
+    for pl, p, v, w in zip(plans, projs, vols, wls):
+        if not g.is_1d(w.scan.kind):
+            continue
+        pl.execute(p, v, stream=stream)
+        arc_ms += min(timed_events(torch, stream, lambda: pl.backproject(v, stream=stream), 2))
+        arc_upd += w.updates
+        arc_bytes += w.scan.num_proj * w.scan.nu * 4 + w.img.nvox * 4
+    roof["fan_arc_slices_sum"] = roofline_entry("fan_arc_1d", arc_upd, arc_ms, arc_bytes)
     out = {
         "workload": f"{n_slices} fan-arc slices (channels U{{300..900}}, views U{{360..1200}}, images U{{256..768}}^2) "
                     f"+ {len(wls) - n_slices} cylindrical slabs (672x64 det, 1152 views, 512x512x64)",
@@ -352,6 +503,7 @@ def run_batched(a, g, W, torch, np):
         "batched_gups": round(upd / t_batch / 1e9, 2),
         "batched_vs_plain": round(t_plain / t_batch, 3),
         "batched_identical_to_plain": same,
+        "roofline": roof,
         "input_generation_s": round(t_gen, 2),
     }
     batch.close()
@@ -491,8 +643,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--sections", default="flat,perview,precision,batched,served,joseph")
-    ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--sections", default="flat,perview,c1,precision,batched,served,joseph")
+    ap.add_argument("--threads", type=int, default=0, help="host threads of the CPU legs (0: host_cores())")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy 5 times (reference kernel for HBM byte counters)")
     a = ap.parse_args()
@@ -502,6 +654,9 @@ def main():
               "large volume by z slab).", file=sys.stderr)
         sys.exit(2)
     sections = {s.strip() for s in a.sections.split(",") if s.strip()}
+    cores, cpu_info = host_cores()
+    if not a.threads:
+        a.threads = cores
 
     import numpy as np
     import torch
@@ -522,6 +677,9 @@ def main():
         return
 
     out, wl, proj, vol = run_flat(a, g, W, torch, np, sections)
+    if "c1" in sections:
+        print("bench: c1 section", file=sys.stderr, flush=True)
+        out["c1_parallel_1d"] = run_c1(a, g, W, torch, np, a.threads)
     if "precision" in sections:
         print("bench: precision section", file=sys.stderr, flush=True)
         out["precision"] = run_precision(a, g, W, torch, np)
@@ -539,7 +697,8 @@ def main():
         out["joseph"] = run_joseph(a, g, W, torch, np)
         torch.cuda.empty_cache()
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(wl, proj, vol, a.threads)
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(wl, proj, vol, a.threads, cpu_info)
     print(json.dumps(out), flush=True)
 
 
