@@ -13,6 +13,8 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstdlib>
+#include <string>
 
 namespace gct {
 
@@ -402,8 +404,22 @@ __global__ void unpack_kernel(const void *src, float *dst, uint32_t nu, uint32_t
 // crossed at t(i) = t0 + i dt; the two in-plane coordinates (in voxel units)
 // are linear in i, so after an FP64 set-up per ray each plane costs two FMAs,
 // two floors and four predicated taps (zero outside the volume).
-__device__ float joseph_ray(const FpLaunch &a, const float o[3], const float d[3], bool from_source)
+// The plane march of one ray: the axis its direction is most aligned with
+// (in voxel units), the planes of voxel centres normal to it that the ray
+// crosses inside the volume grown by one voxel (and, for rays from a source,
+// in front of it), plane i crossed at t(i) = t0 + i dt, and the path length
+// per plane.  ax = -1: the ray misses the volume.
+struct March {
+    int ax, oa, ob;
+    int i0, i1;
+    double t0, dt;
+    float step;
+};
+
+__device__ March joseph_march(const FpLaunch &a, const float o[3], const float d[3], bool from_source)
 {
+    March m{};
+    m.ax = -1;
     const bool is3 = !is_1d(a.kind);
     const float first[3] = {a.x0, a.y0, a.z0};
     const float sp[3] = {a.dx, a.dy, a.dz};
@@ -415,7 +431,7 @@ __device__ float joseph_ray(const FpLaunch &a, const float o[3], const float d[3
         const float q = fabsf(d[i]) / sp[i];
         if (q > best) { best = q; ax = i; }
     }
-    if (!(fabsf(d[ax]) > 0.0f)) return 0.0f;
+    if (!(fabsf(d[ax]) > 0.0f)) return m;
     // clip the plane range to where the ray is inside the (one-voxel grown) box
     float tlo = -3.0e38f, thi = 3.0e38f;
     for (int i = 0; i < dims; ++i) {
@@ -426,16 +442,14 @@ __device__ float joseph_ray(const FpLaunch &a, const float o[3], const float d[3
             tlo = fmaxf(tlo, t0);
             thi = fminf(thi, t1);
         } else if (o[i] < lo || o[i] > hi) {
-            return 0.0f;
+            return m;
         }
     }
     if (from_source) tlo = fmaxf(tlo, 0.0f);
-    if (!(thi > tlo)) return 0.0f;
+    if (!(thi > tlo)) return m;
     const float pa0 = o[ax] + tlo * d[ax], pa1 = o[ax] + thi * d[ax];
     const float ia = (fminf(pa0, pa1) - first[ax]) / sp[ax], ib = (fmaxf(pa0, pa1) - first[ax]) / sp[ax];
     int i0 = max(0, static_cast<int>(floorf(ia))), i1 = min(n[ax] - 1, static_cast<int>(ceilf(ib)));
-    const int oa = (ax == 0) ? 1 : 0, ob = (ax == 2) ? 1 : 2;
-
     const double dax = d[ax];
     const double t0 = (static_cast<double>(first[ax]) - o[ax]) / dax, dt = static_cast<double>(sp[ax]) / dax;
     if (from_source) {   // only planes in front of the source: t(i) > 0
@@ -443,9 +457,51 @@ __device__ float joseph_ray(const FpLaunch &a, const float o[3], const float d[3
         if (dt > 0.0) i0 = max(i0, static_cast<int>(floor(ic)) + 1);
         else i1 = min(i1, static_cast<int>(ceil(ic)) - 1);
     }
-    if (i1 < i0) return 0.0f;
+    if (i1 < i0) return m;
     const float len = sqrtf(d[0] * d[0] + d[1] * d[1] + (is3 ? d[2] * d[2] : 0.0f));
-    const float step = sp[ax] * len / fabsf(d[ax]);
+    m.ax = ax;
+    m.oa = (ax == 0) ? 1 : 0;
+    m.ob = (ax == 2) ? 1 : 2;
+    m.i0 = i0;
+    m.i1 = i1;
+    m.t0 = t0;
+    m.dt = dt;
+    m.step = sp[ax] * len / fabsf(d[ax]);
+    return m;
+}
+
+// In-plane padded coordinates of a ray marching along x or y through the
+// zero-padded copy for its axis: a = the other of x, y, b = z, each linear
+// in the plane index (coordinate + 1, so that clamping to [0, n + 1] lands
+// every out-of-volume tap on zero padding).
+struct PadLine {
+    float fa0p, dfa, fb0p, dfb;
+};
+
+__device__ inline PadLine pad_line(const FpLaunch &a, const March &m, const float o[3], const float d[3])
+{
+    const float first[3] = {a.x0, a.y0, a.z0};
+    const float sp[3] = {a.dx, a.dy, a.dz};
+    const int oa = m.oa;
+    PadLine l;
+    l.fa0p = static_cast<float>((o[oa] + m.t0 * d[oa] - first[oa]) / sp[oa] + 1.0);
+    l.dfa = static_cast<float>(m.dt * d[oa] / sp[oa]);
+    l.fb0p = static_cast<float>((o[2] + m.t0 * d[2] - first[2]) / sp[2] + 1.0);
+    l.dfb = static_cast<float>(m.dt * d[2] / sp[2]);
+    return l;
+}
+
+__device__ float joseph_ray(const FpLaunch &a, const float o[3], const float d[3], bool from_source)
+{
+    const bool is3 = !is_1d(a.kind);
+    const March m = joseph_march(a, o, d, from_source);
+    if (m.ax < 0) return 0.0f;
+    const float first[3] = {a.x0, a.y0, a.z0};
+    const float sp[3] = {a.dx, a.dy, a.dz};
+    const int n[3] = {static_cast<int>(a.nx), static_cast<int>(a.ny), static_cast<int>(is3 ? a.nz : 1u)};
+    const int ax = m.ax, oa = m.oa, ob = m.ob, i0 = m.i0, i1 = m.i1;
+    const double t0 = m.t0, dt = m.dt;
+    const float step = m.step;
     if (is3 && ax != 2 && a.pad != nullptr) {
         // Rays marching along x or y read the zero-padded copy for their axis
         // (joseph_pad_elems): in-plane a = the other of x, y (contiguous in the
@@ -457,10 +513,8 @@ __device__ float joseph_ray(const FpLaunch &a, const float o[3], const float d[3
         const uint32_t na = xm ? a.ny : a.nx, w = na + 3u;
         const uint32_t sb = (xm ? a.nx : a.ny) * w;
         const uint32_t base = xm ? (a.nz + 3u) * (a.ny * (a.nx + 3u)) : 0u;
-        const float fa0p = static_cast<float>((o[oa] + t0 * d[oa] - first[oa]) / sp[oa] + 1.0);
-        const float dfa = static_cast<float>(dt * d[oa] / sp[oa]);
-        const float fb0p = static_cast<float>((o[2] + t0 * d[2] - first[2]) / sp[2] + 1.0);
-        const float dfb = static_cast<float>(dt * d[2] / sp[2]);
+        const PadLine l = pad_line(a, m, o, d);
+        const float fa0p = l.fa0p, dfa = l.dfa, fb0p = l.fb0p, dfb = l.dfb;
         const float nap = static_cast<float>(na + 1u), nbp = static_cast<float>(a.nz + 1u);
         // byte offsets (the copies hold < 2^30 floats): each tap is one load from
         // the uniform base plus a 32-bit offset, the +1 column an immediate
@@ -570,26 +624,15 @@ __global__ __launch_bounds__(256) void pad_xmarch_kernel(const float *__restrict
     }
 }
 
-__global__ __launch_bounds__(256) void joseph_kernel(FpLaunch a)
+// The ray of detector pixel (r, c) in view v: origin o, direction d, and
+// whether only t > 0 counts (rays from a source).
+__device__ inline bool ray_of(const FpLaunch &a, uint32_t v, uint32_t r, uint32_t c, float o[3], float d[3])
 {
-    const size_t idx = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const uint32_t nv = is_1d(a.kind) ? 1u : a.nv;
-    const size_t total = static_cast<size_t>(a.nviews) * nv * a.nu;
-    if (idx >= total) return;
-    // Rays are taken detector row by detector row, all views of a row in
-    // turn: the rays of one row meet the volume in one z slab for every view,
-    // so consecutive workgroups reuse that slab from L2 / MALL instead of
-    // sweeping the whole volume once per view.
-    const uint32_t c = static_cast<uint32_t>(idx % a.nu);
-    const size_t rv = idx / a.nu;
-    const uint32_t v = static_cast<uint32_t>(rv % a.nviews);
-    const uint32_t r = static_cast<uint32_t>(rv / a.nviews);
-    const size_t out = (static_cast<size_t>(v) * nv + r) * a.nu + c;
     const gct_proj_geom g = a.views[v];
     const float uc = (static_cast<float>(c) - g.det_row_center) * a.du;
     const float vr = (static_cast<float>(r) - g.det_col_center) * a.dv;
     const float D = g.src_det_dist;
-    float o[3] = {0.0f, 0.0f, 0.0f}, d[3] = {0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < 3; ++i) o[i] = d[i] = 0.0f;
     bool from_src = true;
     switch (a.kind) {
     case GCT_PARALLEL_1D:
@@ -640,7 +683,281 @@ __global__ __launch_bounds__(256) void joseph_kernel(FpLaunch a)
         break;
     }
     }
+    return from_src;
+}
+
+__global__ __launch_bounds__(256) void joseph_kernel(FpLaunch a)
+{
+    const size_t idx = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint32_t nv = is_1d(a.kind) ? 1u : a.nv;
+    const size_t total = static_cast<size_t>(a.nviews) * nv * a.nu;
+    if (idx >= total) return;
+    // Rays are taken detector row by detector row, all views of a row in
+    // turn: the rays of one row meet the volume in one z slab for every view,
+    // so consecutive workgroups reuse that slab from L2 / MALL instead of
+    // sweeping the whole volume once per view.
+    const uint32_t c = static_cast<uint32_t>(idx % a.nu);
+    const size_t rv = idx / a.nu;
+    const uint32_t v = static_cast<uint32_t>(rv % a.nviews);
+    const uint32_t r = static_cast<uint32_t>(rv / a.nviews);
+    const size_t out = (static_cast<size_t>(v) * nv + r) * a.nu + c;
+    float o[3], d[3];
+    const bool from_src = ray_of(a, v, r, c, o, d);
     a.proj[out] = joseph_ray(a, o, d, from_src);
+}
+
+// ---- Joseph projection through LDS-staged bricks (3-D kinds) ------------------
+//
+// A workgroup traces the rays of kJVB views x kJTR detector rows x kJTC
+// columns together (each thread: one detector pixel in kJVB consecutive
+// views), plane segment by plane segment along the march axis the block
+// shares.  For each segment of kJB planes it takes the bounding box of all
+// its rays' taps in those planes (in the padded copy's in-plane
+// coordinates), stages that brick of the zero-padded copy in LDS -- each
+// voxel is read once per workgroup and segment, where joseph_kernel reads it
+// once per tap, from L2 or beyond -- and takes the bilinear taps from LDS.
+// The views of a block turn by a few tenths of a degree, so their rays
+// through a tile share most of the brick.  Per plane the arithmetic and the
+// order of the sums are joseph_ray's, so each ray's value is the same bit for
+// bit.  Rays whose dominant axis is not the block's (z, or the other in-plane
+// axis near 45 degrees) are traced by joseph_ray afterwards, and a segment
+// whose brick would not fit takes its taps from the copy in global memory.
+constexpr uint32_t kJTC = 32, kJTR = 8, kJVB = 4;   // rays of a workgroup: columns, rows, views
+constexpr uint32_t kJB = 8, kJHM = 16, kJWM = 64;   // brick: planes, at most 16 rows of 64 taps
+constexpr uint32_t kJWP = kJWM + 1u;                // LDS row pitch
+using LdsF = __attribute__((address_space(3))) const float;   // an LDS float, addressed by its 32-bit offset
+
+// 4 workgroups of 33 KB of LDS fit a CU: ask for the registers of 4 waves per SIMD
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+void joseph_lds_kernel(FpLaunch a, uint32_t tiles_c, uint32_t tiles_r)
+{
+    static_assert(kJTC * kJTR == 256, "one detector pixel per thread");
+    __shared__ float brick[kJB * kJHM * kJWP];
+    __shared__ float box[4][4];   // per wave: min a, max a, min b, max b
+    __shared__ int span[2];       // lowest first plane, highest last plane of the block's staged rays
+    __shared__ uint32_t votes[2]; // rays marching along x, along y
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    // view blocks fastest, dealt to the XCDs in contiguous runs: an XCD traces
+    // neighbouring views of one tile in turn, which share most of their bricks
+    const uint32_t nvb = (a.nviews + kJVB - 1u) / kJVB;
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t vb = t % nvb, rest = t / nvb;
+    const uint32_t c = (rest % tiles_c) * kJTC + (tid & (kJTC - 1u));
+    const uint32_t r = (rest / tiles_c) * kJTR + tid / kJTC;
+    const bool pix = c < a.nu && r < a.nv;
+    if (tid == 0) {
+        votes[0] = votes[1] = 0u;
+        span[0] = 0x7FFFFFFF;
+        span[1] = -1;
+    }
+    __syncthreads();
+    // per ray: 0 misses the volume, 1 staged march, 2 joseph_ray afterwards
+    uint32_t mode[kJVB];
+    int axis_of[kJVB], i0[kJVB], i1[kJVB];
+    float fa[kJVB], da[kJVB], fb[kJVB], db[kJVB], step[kJVB], acc[kJVB];
+#pragma unroll
+    for (uint32_t j = 0; j < kJVB; ++j) {
+        mode[j] = 0u;
+        acc[j] = 0.0f;
+        const uint32_t v = vb * kJVB + j;
+        if (!pix || v >= a.nviews) continue;
+        float o[3], d[3];
+        const bool from_src = ray_of(a, v, r, c, o, d);
+        const March m = joseph_march(a, o, d, from_src);
+        if (m.ax < 0) continue;
+        axis_of[j] = m.ax;
+        mode[j] = m.ax == 2 ? 2u : 1u;
+        if (mode[j] == 1u) {
+            atomicAdd(&votes[m.ax], 1u);
+            const PadLine l = pad_line(a, m, o, d);
+            fa[j] = l.fa0p;
+            da[j] = l.dfa;
+            fb[j] = l.fb0p;
+            db[j] = l.dfb;
+            i0[j] = m.i0;
+            i1[j] = m.i1;
+            step[j] = m.step;
+        }
+    }
+    __syncthreads();
+    const int axis = votes[0] >= votes[1] ? 0 : 1;
+#pragma unroll
+    for (uint32_t j = 0; j < kJVB; ++j) {
+        if (mode[j] == 1u && axis_of[j] != axis) mode[j] = 2u;
+        if (mode[j] == 1u) {
+            atomicMin(&span[0], i0[j]);
+            atomicMax(&span[1], i1[j]);
+        }
+    }
+    __syncthreads();
+    const int lo = span[0], hi = span[1];
+    const bool xm = axis == 0;
+    const uint32_t na = xm ? a.ny : a.nx, w = na + 3u;
+    const uint32_t sb = (xm ? a.nx : a.ny) * w;
+    const uint32_t base = xm ? (a.nz + 3u) * (a.ny * (a.nx + 3u)) : 0u;
+    const float nap = static_cast<float>(na + 1u), nbp = static_cast<float>(a.nz + 1u);
+    const float *__restrict__ P = a.pad;
+    const uint32_t brick_lds = static_cast<uint32_t>(reinterpret_cast<uintptr_t>((LdsF *)brick));
+    for (int s0 = lo; s0 <= hi; s0 += static_cast<int>(kJB)) {
+        const int s1 = s0 + static_cast<int>(kJB) - 1;
+        // bounding box of this thread's taps in the segment: the clamped padded
+        // coordinates are monotone in the plane index, so their ends bound them
+        float mina = 3.0e38f, maxa = -1.0f, minb = 3.0e38f, maxb = -1.0f;
+#pragma unroll
+        for (uint32_t j = 0; j < kJVB; ++j) {
+            if (mode[j] != 1u) continue;
+            const int pl = max(s0, i0[j]), ph = min(s1, i1[j]);
+            if (pl > ph) continue;
+            const float a0 = __builtin_amdgcn_fmed3f(fmaf(static_cast<float>(pl), da[j], fa[j]), 0.0f, nap);
+            const float a1 = __builtin_amdgcn_fmed3f(fmaf(static_cast<float>(ph), da[j], fa[j]), 0.0f, nap);
+            const float b0 = __builtin_amdgcn_fmed3f(fmaf(static_cast<float>(pl), db[j], fb[j]), 0.0f, nbp);
+            const float b1 = __builtin_amdgcn_fmed3f(fmaf(static_cast<float>(ph), db[j], fb[j]), 0.0f, nbp);
+            mina = fminf(mina, fminf(a0, a1));
+            maxa = fmaxf(maxa, fmaxf(a0, a1));
+            minb = fminf(minb, fminf(b0, b1));
+            maxb = fmaxf(maxb, fmaxf(b0, b1));
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            mina = fminf(mina, __shfl_xor(mina, off));
+            maxa = fmaxf(maxa, __shfl_xor(maxa, off));
+            minb = fminf(minb, __shfl_xor(minb, off));
+            maxb = fmaxf(maxb, __shfl_xor(maxb, off));
+        }
+        if (lane == 0) {
+            box[wave][0] = mina;
+            box[wave][1] = maxa;
+            box[wave][2] = minb;
+            box[wave][3] = maxb;
+        }
+        __syncthreads();   // also: every thread is done with the previous segment's brick
+        mina = fminf(fminf(box[0][0], box[1][0]), fminf(box[2][0], box[3][0]));
+        maxa = fmaxf(fmaxf(box[0][1], box[1][1]), fmaxf(box[2][1], box[3][1]));
+        minb = fminf(fminf(box[0][2], box[1][2]), fminf(box[2][2], box[3][2]));
+        maxb = fmaxf(fmaxf(box[0][3], box[1][3]), fmaxf(box[2][3], box[3][3]));
+        const bool any = maxa >= 0.0f;   // some ray of the block has taps in this segment
+        const uint32_t alo = any ? static_cast<uint32_t>(mina) : 0u, blo = any ? static_cast<uint32_t>(minb) : 0u;
+        const uint32_t W = any ? static_cast<uint32_t>(maxa) - alo + 2u : 0u;
+        const uint32_t H = any ? static_cast<uint32_t>(maxb) - blo + 2u : 0u;
+        const bool fits = any && W <= kJWM && H <= kJHM;
+        const uint32_t np = static_cast<uint32_t>(min(static_cast<int>(kJB), hi - s0 + 1));
+        if (fits) {
+            // Brick element (p, hb, la) = copy element base + (s0 + p) w + (blo + hb) sb + alo + la:
+            // wave q stages rows q, q + 4, ... of the (plane, row) pairs, one tap per lane.  The
+            // clamped coordinates keep every staged column <= na + 2 and row <= nz + 2, inside the copy.
+            // In two halves: 16 loads in flight per lane, then their stores.
+            constexpr uint32_t kRows = kJB * kJHM / 4u, kHalf = kRows / 2u;
+#pragma unroll
+            for (uint32_t h = 0; h < 2u; ++h) {
+                float val[kHalf];
+#pragma unroll
+                for (uint32_t k = 0; k < kHalf; ++k) {
+                    const uint32_t row = wave + 4u * (h * kHalf + k), p = row / kJHM, hb = row % kJHM;
+                    const bool on = p < np && hb < H && lane < W;
+                    const uint32_t e = base + (static_cast<uint32_t>(s0) + p) * w + (blo + hb) * sb + alo + lane;
+                    val[k] = on ? P[e] : 0.0f;
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < kHalf; ++k) {
+                    const uint32_t row = wave + 4u * (h * kHalf + k), p = row / kJHM, hb = row % kJHM;
+                    if (p < np && hb < H && lane < W) brick[(p * kJHM + hb) * kJWP + lane] = val[k];
+                }
+            }
+        }
+        __syncthreads();
+        if (!any) continue;
+        if (fits) {
+            // LDS byte address of the brick element (0, 0, 0) less that of tap (alo, blo)
+            const uint32_t org = brick_lds - 4u * (blo * kJWP + alo);
+#pragma unroll
+            for (uint32_t j = 0; j < kJVB; ++j) {
+                if (mode[j] != 1u) continue;
+                const int pl = max(s0, i0[j]), ph = min(s1, i1[j]);
+                if (pl > ph) continue;
+                float s = acc[j];
+                const float fs0 = static_cast<float>(s0), fs1 = static_cast<float>(s1);
+                // the unclamped coordinates at both ends: inside [0, n + 1], every plane's
+                // are (they are monotone), and the clamps would return them unchanged
+                const float ea0 = fmaf(fs0, da[j], fa[j]), ea1 = fmaf(fs1, da[j], fa[j]);
+                const float eb0 = fmaf(fs0, db[j], fb[j]), eb1 = fmaf(fs1, db[j], fb[j]);
+                const bool inside = fminf(ea0, ea1) >= 0.0f && fmaxf(ea0, ea1) <= nap && fminf(eb0, eb1) >= 0.0f &&
+                                    fmaxf(eb0, eb1) <= nbp;
+                if (pl == s0 && ph == s1 && inside) {
+                    // the ray crosses the whole segment inside the volume (the common case):
+                    // every tap lies in the box, no plane is masked, no clamp binds
+#pragma unroll
+                    for (uint32_t p = 0; p < kJB; ++p) {
+                        const float fi = fs0 + static_cast<float>(p);   // exact: plane indices < 2^24
+                        const float ca = fmaf(fi, da[j], fa[j]);
+                        const float cb = fmaf(fi, db[j], fb[j]);
+                        const float wa = __builtin_amdgcn_fractf(ca), wb = __builtin_amdgcn_fractf(cb);
+                        // LDS byte address of tap (ia, ib) of plane p: one shift-add and one
+                        // 24-bit multiply-add from the absolute tap indices
+                        const uint32_t ba = __umul24(static_cast<uint32_t>(cb), 4u * kJWP) +
+                                            ((static_cast<uint32_t>(ca) << 2) + org + p * (4u * kJHM * kJWP));
+                        LdsF *q = reinterpret_cast<LdsF *>(static_cast<uintptr_t>(ba));
+                        const float v00 = q[0], v10 = q[1], v01 = q[kJWP], v11 = q[kJWP + 1u];
+                        const float lo2 = fmaf(wa, v10 - v00, v00), hi2 = fmaf(wa, v11 - v01, v01);
+                        s += fmaf(wb, hi2 - lo2, lo2);
+                    }
+                } else {
+#pragma unroll
+                    for (uint32_t p = 0; p < kJB; ++p) {
+                        const int i = s0 + static_cast<int>(p);
+                        const bool in = i >= pl && i <= ph;
+                        const float fi = fs0 + static_cast<float>(p);
+                        const float ca = __builtin_amdgcn_fmed3f(fmaf(fi, da[j], fa[j]), 0.0f, nap);
+                        const float cb = __builtin_amdgcn_fmed3f(fmaf(fi, db[j], fb[j]), 0.0f, nbp);
+                        const float wa = __builtin_amdgcn_fractf(ca), wb = __builtin_amdgcn_fractf(cb);
+                        // masked planes may lie outside the box: the unsigned clamps keep
+                        // their reads inside the brick
+                        const uint32_t la = min(static_cast<uint32_t>(ca) - alo, W - 2u);
+                        const uint32_t lb = min(static_cast<uint32_t>(cb) - blo, H - 2u);
+                        const float *q = brick + p * (kJHM * kJWP) + __umul24(lb, kJWP) + la;
+                        const float v00 = q[0], v10 = q[1], v01 = q[kJWP], v11 = q[kJWP + 1u];
+                        const float lo2 = fmaf(wa, v10 - v00, v00), hi2 = fmaf(wa, v11 - v01, v01);
+                        const float smp = fmaf(wb, hi2 - lo2, lo2);
+                        s = in ? s + smp : s;
+                    }
+                }
+                acc[j] = s;
+            }
+        } else {
+            // the brick would not fit: taps from the copy in global memory, as joseph_ray takes them
+            for (uint32_t j = 0; j < kJVB; ++j) {
+                if (mode[j] != 1u) continue;
+                const int pl = max(s0, i0[j]), ph = min(s1, i1[j]);
+                float s = acc[j];
+                for (int i = pl; i <= ph; ++i) {
+                    const float fi = static_cast<float>(i);
+                    const float ca = __builtin_amdgcn_fmed3f(fmaf(fi, da[j], fa[j]), 0.0f, nap);
+                    const float cb = __builtin_amdgcn_fmed3f(fmaf(fi, db[j], fb[j]), 0.0f, nbp);
+                    const float wa = __builtin_amdgcn_fractf(ca), wb = __builtin_amdgcn_fractf(cb);
+                    const uint32_t e = base + static_cast<uint32_t>(i) * w + __umul24(static_cast<uint32_t>(cb), sb) +
+                                       static_cast<uint32_t>(ca);
+                    const float v00 = P[e], v10 = P[e + 1u], v01 = P[e + sb], v11 = P[e + sb + 1u];
+                    const float lo2 = fmaf(wa, v10 - v00, v00), hi2 = fmaf(wa, v11 - v01, v01);
+                    s += fmaf(wb, hi2 - lo2, lo2);
+                }
+                acc[j] = s;
+            }
+        }
+    }
+    const uint32_t nvr = a.nv;
+#pragma unroll
+    for (uint32_t j = 0; j < kJVB; ++j) {
+        const uint32_t v = vb * kJVB + j;
+        if (!pix || v >= a.nviews) continue;
+        float out = 0.0f;
+        if (mode[j] == 1u) {
+            out = acc[j] * step[j];
+        } else if (mode[j] == 2u) {
+            float o[3], d[3];
+            const bool from_src = ray_of(a, v, r, c, o, d);
+            out = joseph_ray(a, o, d, from_src);
+        }
+        a.proj[(static_cast<size_t>(v) * nvr + r) * a.nu + c] = out;
+    }
 }
 
 // ---- residual of a reprojection -------------------------------------------------
@@ -767,6 +1084,21 @@ void launch_forward_project(const FpLaunch &a, hipStream_t s)
         const uint32_t tx = (a.nx + 31u) / 32u, ty = (a.ny + 3u + 31u) / 32u;
         hipLaunchKernelGGL(pad_xmarch_kernel, dim3(tx * ty * (a.nz + 3u)), dim3(256), 0, s, a.vol,
                            a.pad + static_cast<size_t>(a.nz + 3u) * a.ny * (a.nx + 3u), a.nx, a.ny, a.nz, tx, ty);
+        // GCT_JOSEPH=per_ray selects the one-ray-per-thread kernel instead (the
+        // two give the same bits; the switch is for comparing them)
+        static const bool per_ray = [] {
+            const char *e = std::getenv("GCT_JOSEPH");
+            return e && std::string(e) == "per_ray";
+        }();
+        if (!per_ray) {
+            const uint32_t tiles_c = (a.nu + kJTC - 1u) / kJTC, tiles_r = (a.nv + kJTR - 1u) / kJTR;
+            const uint64_t blocks = uint64_t(tiles_c) * tiles_r * ((a.nviews + kJVB - 1u) / kJVB);
+            if (blocks >= (uint64_t(1) << 31)) fail(GCT_ERR_UNSUPPORTED, "forward projection grid too large");
+            hipLaunchKernelGGL(joseph_lds_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, s, a, tiles_c,
+                               tiles_r);
+            hip_check(hipGetLastError(), "forward projection");
+            return;
+        }
     }
     hipLaunchKernelGGL(joseph_kernel, dim3(static_cast<uint32_t>((total + 255) / 256)), dim3(256), 0, s, a);
     hip_check(hipGetLastError(), "forward projection");

@@ -287,7 +287,10 @@ gct_status gct_plan_store_filtered(gct_plan *plan, float *filt_dev, void *stream
 
 /* Joseph ray-driven forward projection of vol_dev into proj_dev [V][Nv][Nu]
  * along the rays the backprojector's mapping implies.  No reference
- * counterpart (the reference's forward_proj.cuh is never compiled). */
+ * counterpart (the reference's forward_proj.cuh is never compiled).  3-D
+ * kinds trace blocks of rays through volume bricks staged in LDS; the
+ * environment variable GCT_JOSEPH=per_ray (read once per process) selects
+ * the one-ray-per-thread kernel instead, which gives the same bits. */
 gct_status gct_plan_forward_project(gct_plan *plan, const float *vol_dev, float *proj_dev, void *stream);
 
 /* ---- batches: many uneven scans in one graph launch --------------------- */

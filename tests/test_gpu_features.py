@@ -288,6 +288,57 @@ def test_joseph_approximates_analytic_line_integrals(torch_dev, kind):
     assert err < 0.10   # rasterisation error of the phantom; a convention error gives ~1
 
 
+def _joseph_cases():
+    cases = [(k, W.small_case(k, n=40, views=30)) for k in (g.PARALLEL_2D, g.CONE_PARALLEL, g.CYLINDRICAL, g.FLAT)]
+    # tiny voxels under wide detector pixels: 32 columns span > 64 voxels, so every
+    # segment's brick overflows and its taps come from global memory
+    w = W.small_case(g.FLAT, n=48, views=12)
+    cases.append(("flat_wide_bricks", W.Workload("wide", w.scan, g.Image(48, 48, 0.02, 0.02, 48, 0.02), w.phantom)))
+    # flat voxels (dz << dx): steep rays march along z and take the per-ray path
+    cases.append(("flat_z_march", W.Workload("zm", w.scan, g.Image(40, 40, 0.5, 0.5, 40, 0.05), w.phantom)))
+    return cases
+
+
+def test_joseph_lds_bricks_equal_per_ray_kernel(torch_dev, tmp_path):
+    """The LDS-staged projector (workgroups of 4 views x 8 rows x 32 columns,
+    bricks of 8 planes) gives every ray the same bits as the one-ray-per-thread
+    kernel, which a second process selects with GCT_JOSEPH=per_ray."""
+    import subprocess
+    import sys
+    torch = torch_dev
+    cases = _joseph_cases()
+    got = {}
+    rng = np.random.default_rng(5)
+    for name, wl in cases:
+        vol = rng.random(wl.img.shape, dtype=np.float32)   # every tap matters
+        np.save(tmp_path / f"vol_{name}.npy", vol)
+        plan = g.Plan(wl.scan, wl.img)
+        out = torch.empty(wl.scan.proj_shape, dtype=torch.float32, device="cuda")
+        plan.forward_project(torch.from_numpy(vol).cuda(), out)
+        torch.cuda.synchronize()
+        got[name] = out.cpu().numpy()
+        assert np.abs(got[name]).max() > 0, name
+    code = ("import sys, numpy as np, torch; sys.path.insert(0, 'tests')\n"
+            "import gct_amd as g; from test_gpu_features import _joseph_cases\n"
+            f"d = {str(tmp_path)!r}\n"
+            "for name, wl in _joseph_cases():\n"
+            "    vol = np.load(f'{d}/vol_{name}.npy')\n"
+            "    plan = g.Plan(wl.scan, wl.img)\n"
+            "    out = torch.empty(wl.scan.proj_shape, dtype=torch.float32, device='cuda')\n"
+            "    plan.forward_project(torch.from_numpy(vol).cuda(), out); torch.cuda.synchronize()\n"
+            "    np.save(f'{d}/per_ray_{name}.npy', out.cpu().numpy())\n")
+    env = dict(__import__("os").environ, GCT_JOSEPH="per_ray")
+    root = __import__("pathlib").Path(__file__).resolve().parent.parent
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=240, cwd=str(root),
+                       env=env)
+    assert r.returncode == 0, r.stderr[-3000:]
+    for name, _ in cases:
+        ref = np.load(tmp_path / f"per_ray_{name}.npy")
+        diff = int((got[name] != ref).sum())
+        print(f"[joseph lds vs per-ray {name}] {diff} of {ref.size} rays differ", flush=True)
+        assert diff == 0, name
+
+
 @pytest.mark.parametrize("kind", [g.PARALLEL_1D, g.PARALLEL_2D], ids=lambda k: g.KIND_NAMES[k])
 def test_joseph_adjoint_of_backprojection_for_parallel_beams(torch_dev, kind):
     """<A x, y> du ~ <x, B y> dx dy with B the unweighted backprojector: the
