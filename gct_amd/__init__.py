@@ -94,13 +94,17 @@ class PlanInfo(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("device", C.c_int32), ("hbm_budget_bytes", C.c_uint64), ("max_inflight", C.c_uint32),
-                ("num_streams", C.c_uint32)]
+                ("num_streams", C.c_uint32), ("plan_cache_entries", C.c_uint32), ("max_bypass", C.c_uint32)]
 
 
 class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
                 ("rejected", C.c_uint64), ("updates", C.c_uint64), ("peak_device_bytes", C.c_uint64),
-                ("peak_inflight", C.c_uint32), ("busy_seconds", C.c_double)]
+                ("peak_inflight", C.c_uint32), ("busy_seconds", C.c_double), ("plan_cache_hits", C.c_uint64),
+                ("plan_cache_misses", C.c_uint64)]
+
+
+NO_PLAN_CACHE = 0xFFFFFFFF   # gct_server_opts.plan_cache_entries: keep no plans
 
 
 class ArrayMetadata(C.Structure):
@@ -359,6 +363,15 @@ def ramp_kernel(scan: Scan, cutoff: float = 1.0, window: int = WINDOW_NONE) -> n
     taps = np.zeros(scan.nu, np.float32)
     check(lib().gct_ramp_kernel(C.byref(scan.c()), cutoff, window, taps.ctypes.data))
     return taps
+
+
+def required_memory(scan: Scan, img: Image, **kw) -> int:
+    """Device bytes of a reconstruction with every view resident
+    (gct_required_memory): projections, volume and plan -- also a served
+    request's footprint when it asks for no reprojection."""
+    n = C.c_uint64()
+    check(lib().gct_required_memory(C.byref(scan.c()), C.byref(img.c()), C.byref(opts(**kw)), C.byref(n)))
+    return n.value
 
 
 def reconstruct(scan: Scan, proj: np.ndarray, img: Image, **kw) -> np.ndarray:
@@ -638,8 +651,9 @@ def device_count() -> int:
 class Server:
     """Continuous-batching reconstruction server on one device (gct_server)."""
 
-    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, max_inflight: int = 0, num_streams: int = 0):
-        o = ServerOpts(device, hbm_budget_bytes, max_inflight, num_streams)
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, max_inflight: int = 0, num_streams: int = 0,
+                 plan_cache_entries: int = 0, max_bypass: int = 0):
+        o = ServerOpts(device, hbm_budget_bytes, max_inflight, num_streams, plan_cache_entries, max_bypass)
         h = C.c_void_p()
         check(lib().gct_server_create(C.byref(o), C.byref(h)))
         self.h = h

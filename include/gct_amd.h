@@ -348,6 +348,17 @@ typedef struct gct_server_opts {
     uint64_t hbm_budget_bytes;       /* 0: 80 % of free device memory */
     uint32_t max_inflight;           /* requests on the device at once; 0: 8 */
     uint32_t num_streams;            /* HIP streams work is spread over; 0: 4 */
+    /* Plans kept for reuse after their request completes, keyed by the whole
+     * geometry, image grid and options (repeated protocols skip the set-up:
+     * per-view tables, redundancy table, filter spectrum, kernel shape).  Their
+     * device memory counts against hbm_budget_bytes and the least recently
+     * used idle plan is released when a request needs the room.  0: 16;
+     * UINT32_MAX: none. */
+    uint32_t plan_cache_entries;
+    /* Admission is first fit in arrival order; a request passed over this
+     * many times (later requests admitted while it did not fit) blocks every
+     * later request until it is admitted.  0: 4. */
+    uint32_t max_bypass;
 } gct_server_opts;
 
 typedef struct gct_server_stats {
@@ -356,6 +367,7 @@ typedef struct gct_server_stats {
     uint64_t peak_device_bytes;
     uint32_t peak_inflight;
     double busy_seconds;             /* first submit -> last completion */
+    uint64_t plan_cache_hits, plan_cache_misses;
 } gct_server_stats;
 
 gct_status gct_server_create(const gct_server_opts *opts, gct_server **out);

@@ -228,6 +228,78 @@ def test_server_results_identical_alone_and_in_the_mix(torch_dev):
     assert st["peak_inflight"] >= 2
 
 
+def test_server_plan_cache(torch_dev):
+    """A repeated request reuses its cached plan (same bits); a budget that
+    cannot hold a cached plan beside the next request releases the plan, and
+    the server never uses more than its budget."""
+    a = W.small_case(g.FLAT, n=40, views=60)
+    b = W.small_case(g.CONE_PARALLEL, n=48, views=72)
+    pa, pb = a.projections(), b.projections()
+    sv = g.Server(num_streams=2)
+    out1, out2 = np.zeros(a.img.shape, np.float32), np.zeros(a.img.shape, np.float32)
+    sv.wait(sv.submit(a.scan, a.img, pa, out1))
+    sv.wait(sv.submit(a.scan, a.img, pa, out2))
+    st = sv.stats()
+    sv.close()
+    assert (st["plan_cache_misses"], st["plan_cache_hits"]) == (1, 1)
+    assert np.array_equal(out1, out2)
+    assert O.are_equal(out1, O.fdk(a.scan, a.img, pa), 1e-5).all()
+    # room for one request and no more: each plan is released for the next request
+    fa, fb = g.required_memory(a.scan, a.img), g.required_memory(b.scan, b.img)
+    budget = max(fa, fb) + 1024
+    sv = g.Server(num_streams=1, hbm_budget_bytes=budget)
+    outs = []
+    for wl, p in ((a, pa), (b, pb), (a, pa)):
+        o = np.zeros(wl.img.shape, np.float32)
+        sv.wait(sv.submit(wl.scan, wl.img, p, o))
+        outs.append(o)
+    st = sv.stats()
+    sv.close()
+    assert st["peak_device_bytes"] <= budget
+    assert st["plan_cache_hits"] == 0 and st["plan_cache_misses"] == 3
+    assert np.array_equal(outs[0], out1) and np.array_equal(outs[2], out1)
+    assert O.are_equal(outs[1], O.fdk(b.scan, b.img, pb), 1e-5).all()
+
+
+def test_server_admission_ages_bypassed_requests(torch_dev):
+    """First fit with aging: a large request that does not fit while another
+    runs is passed over at most max_bypass (here 1) times; later small
+    requests then wait for it instead of overtaking it indefinitely.  Without
+    the limit every small request finishes before it."""
+    x = W.flat_cone_256(noise=False)                     # ~0.9 GB, ~20 ms with its transfers
+    big = W.serving_mix()[1]                             # the 475^3 short scan, ~2.8 GB
+    small = [W.small_case(g.PARALLEL_1D, n=32, views=40, seed=s) for s in range(5)]
+    px, ps = x.projections_device().cpu().numpy(), [w.projections() for w in small]
+    pbig = big.projections_device().cpu().numpy()
+    fx, fbig = g.required_memory(x.scan, x.img), g.required_memory(big.scan, big.img)
+    fs = max(g.required_memory(w.scan, w.img) for w in small)
+    assert fbig > fx + fs
+    budget = fbig + fs - 1          # the large request only runs alone; x runs beside a small one
+
+    def order(max_bypass):
+        sv = g.Server(num_streams=4, hbm_budget_bytes=budget, plan_cache_entries=g.NO_PLAN_CACHE,
+                      max_bypass=max_bypass)
+        outs = [np.zeros(x.img.shape, np.float32), np.zeros(big.img.shape, np.float32)]
+        outs += [np.zeros(w.img.shape, np.float32) for w in small]
+        import time
+        t0 = time.perf_counter()
+        tickets = [sv.submit(x.scan, x.img, px, outs[0]), sv.submit(big.scan, big.img, pbig, outs[1])]
+        tickets += [sv.submit(w.scan, w.img, p, o) for w, p, o in zip(small, ps, outs[2:])]
+        t_sub = time.perf_counter() - t0
+        done = [sv.wait(t)[0] for t in tickets]   # latency from each submit
+        sv.close()
+        return done, t_sub * 1e3
+
+    done, t_sub = order(1)
+    print(f"\n[aging max_bypass=1] latencies {[round(d, 1) for d in done]} ms, submitted in {t_sub:.1f} ms")
+    assert done[0] > t_sub, "the long request finished before the others were queued"
+    # after one small request passed it, the large one runs before the rest
+    assert sum(1 for d in done[2:] if d < done[1]) <= 1
+    done, _ = order(1000)
+    print(f"[aging max_bypass=1000] latencies {[round(d, 1) for d in done]} ms")
+    assert all(d < done[1] for d in done[2:])
+
+
 def test_server_rejects_requests_over_budget_and_survives_failures(torch_dev):
     w = W.small_case(g.FLAT, n=32, views=60)
     p = w.projections()
