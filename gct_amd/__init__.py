@@ -365,6 +365,25 @@ def ramp_kernel(scan: Scan, cutoff: float = 1.0, window: int = WINDOW_NONE) -> n
     return taps
 
 
+def projection_matrix(view, pix_width: float, pix_height: float) -> np.ndarray:
+    """The 3 x 4 projection matrix of a flat-panel view (a row of Scan.views)
+    normalised by P[2, 3] (ProjectionMatrix(pg, du, dv))."""
+    pg = ProjGeom.from_buffer_copy(np.ascontiguousarray(view, np.float32).tobytes())
+    P = np.zeros((3, 4), np.float64)
+    check(lib().gct_projection_matrix(C.byref(pg), pix_width, pix_height, P.ctypes.data))
+    return P
+
+
+def projection_matrix_view(P: np.ndarray, pix_width: float) -> np.ndarray:
+    """The flat-panel view of a 3 x 4 projection matrix, as a row of
+    Scan.views, with the source-detector distance in mm (the focal length in
+    pixels times pix_width)."""
+    P = np.ascontiguousarray(P, np.float64)
+    pg = ProjGeom()
+    check(lib().gct_projection_matrix_decompose(P.ctypes.data, pix_width, C.byref(pg), None))
+    return np.frombuffer(bytes(pg), np.float32).copy()
+
+
 def required_memory(scan: Scan, img: Image, **kw) -> int:
     """Device bytes of a reconstruction with every view resident
     (gct_required_memory): projections, volume and plan -- also a served

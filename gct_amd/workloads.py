@@ -15,7 +15,7 @@ import numpy as np
 
 from . import (CONE_PARALLEL, CYLINDRICAL, FAN_ARC_1D, FAN_LINE_1D, FLAT, PARALLEL_1D, PARALLEL_2D, Image, Scan,
                WINDOW_NONE, WINDOW_SHEPP_LOGAN, circular_scan, is_1d, phantom_project, phantom_project_device,
-               phantom_random)
+               phantom_random, projection_matrix, projection_matrix_view)
 
 
 @dataclass
@@ -182,6 +182,31 @@ def spiral_case(kind: int = CYLINDRICAL, n: int = 48, rotations: float = 3.0, vi
     ph = phantom_random(seed, 8, 3, 0.25 * n, (0.05, 0.35), (-0.02, 0.1))
     ph[:, 2] += zc
     return Workload(f"spiral_{kind}", scan, img, ph, 0.0, seed)
+
+
+def tilted_flat_case(n: int = 48, views: int = 90, nu: int = 96, nv: int = 80, seed: int = 43) -> Workload:
+    """A flat-panel scan given by projection matrices, as a calibrated C-arm
+    is: the circular views of small_case with every detector turned in its
+    plane by 1.5 deg and tilted about its row axis by 2.5 deg (so the row axis
+    is no longer along z), each view then passed through its 3 x 4 matrix and
+    decomposed back (ProjectionMatrix compose / decompose)."""
+    base = small_case(FLAT, n=n, views=views, nu=nu, nv=nv, seed=seed)
+    sc = base.scan
+    a, b = math.radians(1.5), math.radians(2.5)
+    out = np.empty_like(sc.views)
+    for i, row in enumerate(sc.views):
+        v = row.astype(np.float64)
+        eu, ev = v[8:11], v[11:14]
+        ew = np.cross(eu, ev)
+        eu1 = math.cos(a) * eu + math.sin(a) * ev           # in-plane turn
+        ev1 = -math.sin(a) * eu + math.cos(a) * ev
+        ev2 = math.cos(b) * ev1 + math.sin(b) * ew          # tilt about the row axis
+        t = v.copy()
+        t[8:11], t[11:14] = eu1, ev2
+        P = projection_matrix(t.astype(np.float32), sc.du, sc.dv)
+        out[i] = projection_matrix_view(P, sc.du)
+    scan = Scan(FLAT, sc.nu, sc.du, sc.nv, sc.dv, out)
+    return Workload("tilted_flat", scan, base.img, base.phantom, 0.0, seed)
 
 
 # The reference's end-to-end FDK regression geometries (test/cuda/recon/

@@ -310,9 +310,101 @@ struct ConeBeamProjGeomParams {
     Units::DetPixels det_col_center;
 };
 
+// ProjectionMatrix (src/cpp/geom/projection_matrix.h:14-72): a flat-panel
+// view as a 3x4 matrix P = K [R | -R s] normalised by P(2,3), composed from a
+// view and the pixel sizes (projection_matrix.cpp:66-96) or decomposed into
+// the intrinsic matrix K (positive diagonal, K(2,2) = 1), the rotation R and
+// the source position s (:4-65).  The reference holds Eigen matrices; here
+// they are row-major arrays of the same FP32 values.
+class ProjectionMatrix {
+public:
+    using Mat34 = std::array<std::array<float, 4>, 3>;
+    using Mat33 = std::array<std::array<float, 3>, 3>;
+
+    explicit ProjectionMatrix(ProjGeom<DetFanLine, DetFanLine> const &pg, float pix_width_mm, float pix_height_mm)
+    {
+        const gct_proj_geom g = Detail::toC(pg);
+        double p[12];
+        GCT::Detail::check(gct_projection_matrix(&g, pix_width_mm, pix_height_mm, p));
+        set(p);
+        focal_ = pg.src_det_dist.val / pix_width_mm;
+        ppu_ = pg.det_row_center.val;
+        ppv_ = pg.det_col_center.val;
+        intr_ = {{{focal_, 0.0f, ppu_}, {0.0f, pg.src_det_dist.val / pix_height_mm, ppv_}, {0.0f, 0.0f, 1.0f}}};
+        rot_ = {{{g.det_row_dir[0], g.det_row_dir[1], g.det_row_dir[2]},
+                 {g.det_col_dir[0], g.det_col_dir[1], g.det_col_dir[2]},
+                 {g.det_row_dir[1] * g.det_col_dir[2] - g.det_row_dir[2] * g.det_col_dir[1],
+                  g.det_row_dir[2] * g.det_col_dir[0] - g.det_row_dir[0] * g.det_col_dir[2],
+                  g.det_row_dir[0] * g.det_col_dir[1] - g.det_row_dir[1] * g.det_col_dir[0]}}};
+        src_ = {g.src_pos[0], g.src_pos[1], g.src_pos[2]};
+    }
+    explicit ProjectionMatrix(std::array<double, 12u> const &mat)
+    {
+        gct_proj_geom g{};
+        double f[2];
+        GCT::Detail::check(gct_projection_matrix_decompose(mat.data(), 1.0f, &g, f));
+        double p[12];
+        std::copy(mat.begin(), mat.end(), p);
+        set(p);
+        focal_ = static_cast<float>(f[0]);
+        ppu_ = g.det_row_center;
+        ppv_ = g.det_col_center;
+        rot_ = {{{g.det_row_dir[0], g.det_row_dir[1], g.det_row_dir[2]},
+                 {g.det_col_dir[0], g.det_col_dir[1], g.det_col_dir[2]},
+                 {g.iso_ray_dir[0], g.iso_ray_dir[1], g.iso_ray_dir[2]}}};
+        src_ = {g.src_pos[0], g.src_pos[1], g.src_pos[2]};
+        // K = M R^T (M the left 3x3 of P), normalised so that K(2,2) = 1
+        double k[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                k[i][j] = mat[i * 4 + 0] * rot_[j][0] + mat[i * 4 + 1] * rot_[j][1] + mat[i * 4 + 2] * rot_[j][2];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) intr_[i][j] = static_cast<float>(j < i ? 0.0 : k[i][j] / k[2][2]);
+    }
+
+    Mat34 matrix() const noexcept { return mat_; }
+    float principalPointDetRow() const noexcept { return ppu_; }
+    float principalPointDetCol() const noexcept { return ppv_; }
+    // focal length (source-detector distance) in pixels of a detector row
+    float focalLengthDetRow() const noexcept { return focal_; }
+    std::array<float, 3> srcPos() const noexcept { return src_; }
+    Mat33 rotationMatrix() const noexcept { return rot_; }
+    Mat33 intrinsicMatrix() const noexcept { return intr_; }
+
+private:
+    void set(const double *p)
+    {
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 4; ++j) mat_[i][j] = static_cast<float>(p[i * 4 + j]);
+    }
+    Mat34 mat_{};
+    Mat33 intr_{}, rot_{};
+    float ppu_ = 0.0f, ppv_ = 0.0f, focal_ = 0.0f;
+    std::array<float, 3> src_{};
+};
+
 // ProjGeomFactory<...>::create (src/cpp/io/geom_reader/proj_geom_factory_params.cpp:8-108).
 template <typename DetRow, typename... DetCol>
 struct ProjGeomFactory {
+    // create(ProjectionMatrix) (proj_geom_factory_proj_mat.cpp:7-26), flat panels
+    // only: as in the reference the source-detector distance is the focal
+    // length in pixels of a detector row -- the caller multiplies it by the
+    // pixel width (the reference's TODO at :13)
+    static ProjGeom<DetRow, DetCol...> create(ProjectionMatrix const &m)
+        requires(std::is_same_v<ProjGeom<DetRow, DetCol...>, ProjGeom<DetFanLine, DetFanLine>>)
+    {
+        ProjGeom<DetFanLine, DetFanLine> pg{};
+        pg.src_det_dist.val = m.focalLengthDetRow();
+        pg.det_row_center.val = m.principalPointDetRow();
+        pg.det_col_center.val = m.principalPointDetCol();
+        const auto s = m.srcPos();
+        const auto r = m.rotationMatrix();
+        pg.src_pos = VecWCS<3u>{{s[0]}, {s[1]}, {s[2]}};
+        pg.det_row_dir = VecWCS<3u>{{r[0][0]}, {r[0][1]}, {r[0][2]}};
+        pg.det_col_dir = VecWCS<3u>{{r[1][0]}, {r[1][1]}, {r[1][2]}};
+        return pg;
+    }
+
     template <typename Params>
     static ProjGeom<DetRow, DetCol...> create(Params const &p, unsigned proj_idx)
     {
@@ -700,6 +792,31 @@ struct ProjectionReader {
             p.setData(std::move(pix));
             return p;
         }
+    }
+};
+
+// ProjMatReader (proj_mat_reader.h:9-45): one 3x4 projection matrix (12
+// elements, row-major, read as double) per slice of the file or series.
+template <typename FileReader>
+struct ProjMatReader {
+    static std::vector<Geom::ProjectionMatrix> read(std::string const &filename, bool series = false)
+    {
+        const FileReader reader{filename, series};
+        const ArrayMetadata md = reader.readMetadata();
+        const std::vector<double> all = reader.template readData<double>();
+        constexpr size_t kElems = 12u;
+        const size_t num_mats = md.num_slices;
+        if (all.size() != kElems * num_mats)
+            throw std::runtime_error{"Error during the reading of projection matrix file " + filename + ".\n" +
+                                     std::string{"There should be 12 elements in each projection matrix.\n"}};
+        std::vector<Geom::ProjectionMatrix> out;
+        out.reserve(num_mats);
+        std::array<double, kElems> m{};
+        for (size_t i = 0; i < num_mats; ++i) {
+            std::copy_n(all.begin() + static_cast<std::ptrdiff_t>(i * kElems), kElems, m.begin());
+            out.emplace_back(m);
+        }
+        return out;
     }
 };
 

@@ -201,6 +201,60 @@ int main(int argc, char **argv)
         CHECK(dm.num_rows == 2u && dm.num_cols == 3u && dm.num_slices == 1u && dm.type == IO::ArrayElemType::T_FLOAT);
         fs::remove_all(dir);
     }
+    // projection matrices (proj_geom_factory_proj_mat_test.cpp:16-45): the C-arm matrix
+    // decomposed, its view made by the factory (focal length in pixels, times 0.616 mm),
+    // and composed back; then two matrices through a MetaImage file and ProjMatReader
+    {
+        const std::array<double, 12> artis = {2.48159483828274, 0.39171974522293, 0.0, 307.5,
+                                              0.0, 0.305095541401274, -2.48159483828274, 239.5,
+                                              0.0, 0.00127388535031847, 0.0, 1.0};
+        constexpr float pix = 0.616f;
+        const ProjectionMatrix dec{artis};
+        auto pg = ProjGeomFactory<DetFanLine, DetFanLine>::create(dec);
+        pg.src_det_dist.val *= pix;
+        const float ang = std::atan2(pg.src_pos.y.val, pg.src_pos.x.val);
+        CHECK(near(ang < 0.0f ? ang + 2.0f * std::numbers::pi_v<float> : ang, 3.0f * std::numbers::pi_v<float> / 2.0f, 1e-6f));
+        CHECK(near(pg.src_det_dist.val, 1200.0f, 1e-3f));
+        CHECK(pg.det_row_center.val == 307.5f && pg.det_col_center.val == 239.5f);
+        CHECK(near(pg.det_row_dir.x.val, 1.0f, 1e-8f) && near(pg.det_row_dir.y.val, 0.0f, 1e-8f) &&
+              near(pg.det_row_dir.z.val, 0.0f, 1e-8f));
+        CHECK(near(pg.det_col_dir.x.val, 0.0f, 1e-8f) && near(pg.det_col_dir.y.val, 0.0f, 1e-8f) &&
+              near(pg.det_col_dir.z.val, -1.0f, 1e-8f));
+        const ProjectionMatrix comp{pg, pix, pix};
+        CHECK(near(dec.focalLengthDetRow(), comp.focalLengthDetRow(), 1e-2f));
+        CHECK(near(dec.principalPointDetRow(), comp.principalPointDetRow()) &&
+              near(dec.principalPointDetCol(), comp.principalPointDetCol()));
+        const auto K = dec.intrinsicMatrix();
+        CHECK(near(K[0][0], dec.focalLengthDetRow(), 1e-2f) && K[2][2] == 1.0f && K[1][0] == 0.0f);
+        const auto M = comp.matrix();
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 4; ++j) CHECK(near(M[i][j], float(artis[i * 4 + j]), 1e-4f * (1.0f + std::fabs(float(artis[i * 4 + j])))));
+
+        namespace fs = std::filesystem;
+        const fs::path dir = fs::temp_directory_path() / ("gct_pmat_" + std::to_string(::getpid()));
+        fs::create_directories(dir);
+        std::vector<double> two(artis.begin(), artis.end());
+        for (int j = 0; j < 12; ++j) two.push_back(artis[j] * 2.0);   // the same view, scaled matrix
+        IO::ArrayFileWriter::MetaImgMetaData<3u> md;
+        md.dim_size = {4u, 3u, 2u};
+        const std::string f = (dir / "pmats.mha").string();
+        IO::ArrayFileWriter::write(md, two, f);
+        const auto mats = IO::ProjMatReader<IO::MetaImgFileReader>::read(f);
+        CHECK(mats.size() == 2u);
+        CHECK(mats.size() == 2u && near(mats[1].focalLengthDetRow(), dec.focalLengthDetRow(), 1e-2f) &&
+              near(mats[1].principalPointDetRow(), 307.5f));
+        md.dim_size = {11u, 1u, 2u};   // 11 elements per slice: not a projection matrix
+        const std::string bad = (dir / "bad.mha").string();
+        IO::ArrayFileWriter::write(md, std::vector<double>(22, 1.0), bad);
+        bool threw = false;
+        try {
+            (void)IO::ProjMatReader<IO::MetaImgFileReader>::read(bad);
+        } catch (std::runtime_error const &e) {
+            threw = std::string(e.what()).find("12 elements") != std::string::npos;
+        }
+        CHECK(threw);
+        fs::remove_all(dir);
+    }
 
     if (argc >= 3 && std::strcmp(argv[1], "--gpu") == 0) {
         // flat-panel full scan through the drop-in call, 48^3 from a uniform sphere's exact projections

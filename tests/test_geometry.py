@@ -241,3 +241,21 @@ def test_phantom_random_is_seeded_and_inside_fov():
     assert np.array_equal(a, b)
     assert np.all(np.linalg.norm(a[:, :3], axis=1) <= 0.8 * 32.0 + 1e-9)
     assert np.all((a[:, 3:6] >= 0.05 * 32) & (a[:, 3:6] <= 0.35 * 32))
+
+
+def test_projection_matrix_round_trip_of_tilted_views():
+    """Compose then decompose the views of a scan whose detectors are turned
+    and tilted (the projection-matrix input the tilted-detector GPU test
+    reconstructs): centres, source, row and column directions come back."""
+    from gct_amd import workloads as W
+    base = W.small_case(g.FLAT, n=48, views=90, nu=96, nv=80, seed=43)
+    wl = W.tilted_flat_case()
+    for i in (0, 17, 45, 89):
+        v = wl.scan.views[i]
+        back = g.projection_matrix_view(g.projection_matrix(v, wl.scan.du, wl.scan.dv), wl.scan.du)
+        np.testing.assert_allclose(back[1], v[1], rtol=1e-6)            # source-detector distance
+        np.testing.assert_allclose(back[3:5], v[3:5], atol=1e-4)        # principal point
+        np.testing.assert_allclose(back[5:14], v[5:14], atol=2e-5)      # source, row and column directions
+        # the tilt is real: the row axis leaves the xy-plane, the source is the circular one
+        assert abs(v[10]) > 0.02 and abs(v[13]) < 0.999
+        np.testing.assert_allclose(v[5:8], base.scan.views[i][5:8], atol=1e-3)
