@@ -31,7 +31,7 @@ STORE_F32, STORE_F16, STORE_BF16 = range(3)
 # gct_setup_precision
 SETUP_AUTO, SETUP_F32, SETUP_F64 = range(3)
 # gct_mem_mode
-MEM_AUTO, MEM_HIGH, MEM_LOW = range(3)
+MEM_AUTO, MEM_HIGH, MEM_LOW, MEM_SLAB = range(4)
 # gct_bp_kernel
 BP_AUTO, BP_DIRECT, BP_LDS = range(3)
 
@@ -74,7 +74,8 @@ class ReconOpts(C.Structure):
     _fields_ = [("cut_off_freq", C.c_float), ("window", C.c_int32), ("storage", C.c_int32),
                 ("setup_precision", C.c_int32), ("mem_mode", C.c_int32), ("device", C.c_int32),
                 ("z_unroll", C.c_uint32), ("view_chunk", C.c_uint32), ("bp_kernel", C.c_int32),
-                ("view_block", C.c_uint32), ("lds_rows", C.c_uint32), ("lds_pairs", C.c_uint32)]
+                ("view_block", C.c_uint32), ("lds_rows", C.c_uint32), ("lds_pairs", C.c_uint32),
+                ("z_slab", C.c_uint32)]
 
 
 class CircularParams(C.Structure):

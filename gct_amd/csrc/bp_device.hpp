@@ -286,11 +286,11 @@ __device__ inline void bp_tile3d(const BpLaunch &a, uint32_t bx, uint32_t by, ui
     using T = typename Stor<S>::T;
     const uint32_t ix = bx * 16u + (tid & 15u);
     const uint32_t iy = by * 16u + (tid >> 4);
-    const uint32_t iz0 = bz * ZU;
+    const uint32_t iz0 = a.z0 + bz * ZU;   // global slice index (the launch may cover a z slab)
     if (ix >= a.nx || iy >= a.ny) return;
-    const uint32_t zn = min(static_cast<uint32_t>(ZU), a.nz - iz0);
+    const uint32_t zn = min(static_cast<uint32_t>(ZU), a.z0 + a.zcount - iz0);
     const size_t slice = static_cast<size_t>(a.nx) * a.ny;
-    float *vp = a.vol + static_cast<size_t>(iz0) * slice + static_cast<size_t>(iy) * a.nx + ix;
+    float *vp = a.vol + static_cast<size_t>(iz0 - a.z0) * slice + static_cast<size_t>(iy) * a.nx + ix;
 
     float acc[ZU];
 #pragma unroll
@@ -474,7 +474,7 @@ __device__ inline void tile_boxes(const BpLaunch &a, uint32_t bx, uint32_t by, u
     const uint32_t j = tid >> 3, corner = tid & 7u;
     const uint32_t cx = min(bx * 16u + ((corner & 1u) ? 15u : 0u), a.nx - 1u);
     const uint32_t cy = min(by * 16u + ((corner & 2u) ? 15u : 0u), a.ny - 1u);
-    const uint32_t czt = bz * ZU + ((corner & 4u) ? static_cast<uint32_t>(ZU - 1) : 0u);
+    const uint32_t czt = a.z0 + bz * ZU + ((corner & 4u) ? static_cast<uint32_t>(ZU - 1) : 0u);
     const uint32_t cz = PADZ ? czt : min(czt, a.nz - 1u);
     const float nuf = static_cast<float>(a.nu);
     const ColSetup s = setup_column<KIND, false, false>(a, vb0 + j, static_cast<float>(cx), static_cast<float>(cy), 0.0f, nuf);
@@ -628,11 +628,13 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
     // 4 x 4 block: the lanes' detector columns then span about a quarter as
     // many LDS banks per view as a 16-voxel strip's do.
     const uint32_t ix = bx * 16u + ((tid >> 6) & 1u) * 8u + ((tid >> 4) & 1u) * 4u + (tid & 3u);
-    const uint32_t iy = by * 16u + (tid >> 7) * 8u + ((tid >> 5) & 1u) * 4u + ((tid >> 2) & 3u), iz0 = bz * ZU;
+    const uint32_t iy = by * 16u + (tid >> 7) * 8u + ((tid >> 5) & 1u) * 4u + ((tid >> 2) & 3u);
+    const uint32_t iz0 = a.z0 + bz * ZU;   // global slice index (the launch may cover a z slab)
     const bool active = ix < a.nx && iy < a.ny;
-    const uint32_t zn = min(static_cast<uint32_t>(ZU), a.nz - iz0);
+    const uint32_t zn = min(static_cast<uint32_t>(ZU), a.z0 + a.zcount - iz0);
     const size_t slice = static_cast<size_t>(a.nx) * a.ny;
-    float *vp = a.vol + static_cast<size_t>(iz0) * slice + static_cast<size_t>(active ? iy : 0u) * a.nx + (active ? ix : 0u);
+    float *vp = a.vol + static_cast<size_t>(iz0 - a.z0) * slice + static_cast<size_t>(active ? iy : 0u) * a.nx +
+                (active ? ix : 0u);
 
     float acc[ZU];
 #pragma unroll
@@ -972,7 +974,7 @@ void launch_bp3d_zu(const BpLaunch &a, hipStream_t s)
 {
     const uint32_t nbx = (a.nx + 15u) / 16u, nby = (a.ny + 15u) / 16u;
     const uint32_t zu = a.z_unroll;
-    const uint32_t nbz = (a.nz + zu - 1u) / zu;
+    const uint32_t nbz = (a.zcount + zu - 1u) / zu;
     const uint32_t blocks = nbx * nby * nbz;
     switch (zu) {
     case 4: hipLaunchKernelGGL((bp3d_kernel<KIND, S, CIRC, F64, 4>), dim3(blocks), dim3(256), 0, s, a); break;
@@ -1003,14 +1005,14 @@ void launch_bp3d_lds(const BpLaunch &a, hipStream_t s)
     if constexpr (KIND == GCT_FLAT || KIND == GCT_CYLINDRICAL) {
         if (a.zwin) {   // spiral: per-slice view windows, default shape
             hipLaunchKernelGGL((bp3d_lds_kernel<KIND, S, F64, 16, 40, 2, 40, true>),
-                               dim3(nbx * nby * ((a.nz + 15u) / 16u)), b, 0, s, a);
+                               dim3(nbx * nby * ((a.zcount + 15u) / 16u)), b, 0, s, a);
             return;
         }
     }
     // the plan resolved (z_unroll, view_block, lds_rows, lds_pairs) to a row of kLdsShapes
     const int shape = lds_shape(a.z_unroll, a.view_block, a.lds_rows, a.lds_pairs);
     const uint32_t zu = shape >= 0 ? kLdsShapes[shape].zu : 16u;
-    const dim3 g(nbx * nby * ((a.nz + zu - 1u) / zu));
+    const dim3 g(nbx * nby * ((a.zcount + zu - 1u) / zu));
     static_assert(kLdsShapes[0].zu == 16 && kLdsShapes[0].vb == 2 && kLdsShapes[0].rows == 40 && kLdsShapes[0].pairs == 40);
     static_assert(kLdsShapes[1].zu == 16 && kLdsShapes[1].vb == 2 && kLdsShapes[1].rows == 32 && kLdsShapes[1].pairs == 40);
     static_assert(kLdsShapes[2].zu == 16 && kLdsShapes[2].vb == 2 && kLdsShapes[2].rows == 40 && kLdsShapes[2].pairs == 48);

@@ -157,6 +157,10 @@ struct BpLaunch {
     const uint32_t *zwin;         // spiral: [first, last) view window per z slice (device); null otherwise
     bool rows_inside;             // every voxel's row lies inside the detector: LDS kernel without row clamp
     bool round_robin;             // LDS kernel: tiles dealt to the XCDs round robin, not in contiguous runs
+    // The launch covers slices [z0, z0 + zcount) of the nz-slice volume and `vol`
+    // holds just those (a z slab; z0 a multiple of 32, so tiles and their boxes
+    // are the whole volume's).  Whole volume: z0 = 0, zcount = nz.
+    uint32_t z0, zcount;
 };
 
 // Shapes of the LDS backprojection kernel: voxels per thread along z, views
@@ -240,7 +244,7 @@ __host__ __device__ inline uint32_t batch_tile_depth(int kind, int circular)
 }
 
 // One work item of the persistent batched backprojection: a z-block of one scan.
-struct BatchScan {
+struct alignas(16) BatchScan {
     BpLaunch bp;                  // per-scan parameters (views: all)
     uint32_t tiles_x, tiles_y, tiles_z;
     uint32_t tile_begin;          // prefix sum over scans

@@ -621,6 +621,8 @@ BpLaunch bp_args(gct_plan &p, float *vol, uint32_t first, uint32_t n, bool accum
     a.nx = p.img.x.num_samples;
     a.ny = p.img.y.num_samples;
     a.nz = (p.img.dim == 3) ? p.img.z.num_samples : 1u;
+    a.z0 = 0;
+    a.zcount = a.nz;
     a.nu = p.nu;
     a.nv = p.nv;
     a.first_view = first;
@@ -885,10 +887,171 @@ extern "C" gct_status gct_required_memory(const gct_ct_geom *geom, const gct_ima
     });
 }
 
+namespace {
+
+struct Stream {
+    hipStream_t s = nullptr;
+    Stream() { hip_check(hipStreamCreate(&s), "hipStreamCreate"); }
+    ~Stream() { (void)hipStreamDestroy(s); }
+    Stream(const Stream &) = delete;
+    Stream &operator=(const Stream &) = delete;
+};
+
+struct Event {
+    hipEvent_t e = nullptr;
+    Event() { hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate"); }
+    ~Event() { (void)hipEventDestroy(e); }
+    Event(const Event &) = delete;
+    Event &operator=(const Event &) = delete;
+};
+
+// Two device staging buffers for chunks of the host projection stack: the
+// copy of one chunk (on a stream of its own) overlaps the work queued on the
+// other.  stage() returns the buffer the work must read; release() marks that
+// work as queued, after which the buffer is refilled only once it is done.
+class Stager {
+public:
+    explicit Stager(size_t bytes) : buf_{DevBuf(bytes), DevBuf(bytes)} {}
+    ~Stager() { (void)hipStreamSynchronize(cs_.s); }
+    const float *stage(const float *src, size_t bytes, hipStream_t s)
+    {
+        if (used_[idx_]) hip_check(hipStreamWaitEvent(cs_.s, done_[idx_].e, 0), "wait");
+        hip_check(hipMemcpyAsync(buf_[idx_].p, src, bytes, hipMemcpyHostToDevice, cs_.s), "copy projection chunk");
+        hip_check(hipEventRecord(copied_[idx_].e, cs_.s), "record");
+        hip_check(hipStreamWaitEvent(s, copied_[idx_].e, 0), "wait");
+        return buf_[idx_].as<float>();
+    }
+    void release(hipStream_t s)
+    {
+        hip_check(hipEventRecord(done_[idx_].e, s), "record");
+        used_[idx_] = true;
+        idx_ ^= 1u;
+    }
+
+private:
+    DevBuf buf_[2];
+    Stream cs_;
+    Event copied_[2], done_[2];
+    bool used_[2] = {false, false};
+    uint32_t idx_ = 0;
+};
+
+// Views per chunk when the stack streams through: about 1/16 of free memory
+// for a chunk's staging and filtered views.  The filter transforms detector
+// rows in pairs (2k, 2k + 1) counted from the launch's first view, and a
+// pair's rounding depends on both rows.  Every chunk must start at an even
+// row, as the whole stack's launch does, so that chunked and whole-stack
+// filtering pair (and round) alike: with an odd Nv, an even number of views
+// per chunk.
+uint32_t view_chunk(const gct_ct_geom &geom, const gct_recon_opts &o, size_t free_b, uint64_t view_bytes)
+{
+    const uint64_t nv = is_1d(geom.kind) ? 1u : geom.det_col.num_samples;
+    uint32_t chunk = o.view_chunk;
+    if (!chunk) {
+        const uint64_t per = view_bytes + (nv + 3u) * ((geom.det_row.num_samples + 18u) & ~15u) * storage_bytes(o.storage);
+        chunk = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(geom.num_proj, (free_b / 16) / per)));
+    }
+    chunk = std::min(chunk, geom.num_proj);
+    if ((nv & 1u) && (chunk & 1u) && chunk < geom.num_proj) chunk = chunk > 1u ? chunk - 1u : 2u;
+    return chunk;
+}
+
+// Filters and backprojects views [0, V) into `vol` (slices [z0, z0 + zcount)),
+// chunk by chunk through the stager: each voxel adds its views in scan order.
+void stream_views(gct_plan &plan, const float *proj, uint64_t view_bytes, uint32_t chunk, Stager &st, float *vol,
+                  uint32_t z0, uint32_t zcount, hipStream_t s)
+{
+    for (uint32_t first = 0; first < plan.V; first += chunk) {
+        const uint32_t n = std::min(chunk, plan.V - first);
+        const float *d = st.stage(proj + size_t(first) * view_bytes / sizeof(float), size_t(n) * view_bytes, s);
+        launch_filter(filter_args(plan, d, first, first, n, first), s);
+        BpLaunch a = bp_args(plan, vol, first, n, first > 0, first + n >= plan.V, first);
+        a.z0 = z0;
+        a.zcount = zcount;
+        launch_backproject(a, s);
+        st.release(s);
+    }
+}
+
+// Volumes beyond device memory: the volume is reconstructed in z slabs of
+// `zslab` slices (a multiple of 32, so every backprojection tile, and the
+// detector box it stages, is the whole volume's), two slab buffers in turn,
+// the copy of slab k back to the host overlapping the backprojection of
+// slab k + 1.  The filtered stack is built once and kept when it fits
+// (`resident`); otherwise every slab streams the views through again.  A
+// voxel's sum is the same either way as in the whole-volume paths.
+void reconstruct_slabs(const gct_ct_geom &geom, const float *proj, const gct_image_geom &img, float *vol,
+                       const gct_recon_opts &o, size_t free_b, uint64_t view_bytes)
+{
+    const uint32_t nz = img.z.num_samples;
+    const uint64_t slice = uint64_t(img.x.num_samples) * img.y.num_samples;
+    const uint32_t chunk = view_chunk(geom, o, free_b, view_bytes);
+    const uint64_t stage_b = 2u * uint64_t(chunk) * view_bytes;
+    // explicit view_chunk: the views stream per slab; otherwise the whole
+    // filtered stack stays resident when it takes at most half of free memory
+    const bool resident = !o.view_chunk && plan_device_bytes(geom, img, o, 0) + stage_b <= free_b / 2;
+    const uint64_t fixed = plan_device_bytes(geom, img, o, resident ? 0u : chunk) + stage_b;
+    uint32_t zslab = o.z_slab;
+    if (!zslab) {
+        const uint64_t avail = free_b > fixed ? (free_b - fixed) / 10u * 8u : 0u;
+        zslab = static_cast<uint32_t>(std::min<uint64_t>(nz, avail / (2u * slice * sizeof(float))) & ~31u);
+        if (!zslab) fail(GCT_ERR_OUT_OF_MEMORY, "device memory holds no 32-slice slab beside the filtered views");
+    }
+    zslab = std::min(zslab, nz);
+    auto plan = make_plan(geom, img, o, resident ? geom.num_proj : chunk);
+    Stream s, cs;
+    Event bp_done[2], copied[2];
+    Stager st(size_t(chunk) * view_bytes);
+    if (resident) {
+        for (uint32_t first = 0; first < plan->V; first += chunk) {
+            const uint32_t n = std::min(chunk, plan->V - first);
+            const float *d = st.stage(proj + size_t(first) * view_bytes / sizeof(float), size_t(n) * view_bytes, s.s);
+            launch_filter(filter_args(*plan, d, first, first, n, 0), s.s);
+            st.release(s.s);
+        }
+    }
+    DevBuf slab[2] = {DevBuf(size_t(zslab) * slice * sizeof(float)), DevBuf(size_t(zslab) * slice * sizeof(float))};
+    // The copy of slab k - 1 is queued after the backprojection of slab k, so
+    // that the two overlap also when a copy to pageable memory holds the host.
+    auto copy_back = [&](uint32_t k) {
+        const uint32_t z0 = k * zslab, zc = std::min(zslab, nz - z0);
+        hip_check(hipStreamWaitEvent(cs.s, bp_done[k & 1u].e, 0), "wait");
+        hip_check(hipMemcpyAsync(vol + size_t(z0) * slice, slab[k & 1u].p, size_t(zc) * slice * sizeof(float),
+                                 hipMemcpyDeviceToHost, cs.s),
+                  "copy slab");
+        hip_check(hipEventRecord(copied[k & 1u].e, cs.s), "record");
+    };
+    const uint32_t nslab = (nz + zslab - 1u) / zslab;
+    for (uint32_t k = 0; k < nslab; ++k) {
+        const uint32_t z0 = k * zslab, zc = std::min(zslab, nz - z0);
+        // slab buffer k & 1 was last copied out by copy_back(k - 2); the
+        // backprojection waits for that copy
+        if (k >= 2) hip_check(hipStreamWaitEvent(s.s, copied[k & 1u].e, 0), "wait");
+        float *dv = slab[k & 1u].as<float>();
+        if (resident) {
+            BpLaunch a = bp_args(*plan, dv, 0, plan->V, false, true, 0);
+            a.z0 = z0;
+            a.zcount = zc;
+            launch_backproject(a, s.s);
+        } else {
+            stream_views(*plan, proj, view_bytes, chunk, st, dv, z0, zc, s.s);
+        }
+        hip_check(hipEventRecord(bp_done[k & 1u].e, s.s), "record");
+        if (k >= 1) copy_back(k - 1);
+    }
+    copy_back(nslab - 1u);
+    hip_check(hipStreamSynchronize(cs.s), "copy slabs");
+    hip_check(hipStreamSynchronize(s.s), "reconstruct slabs");
+}
+
+}  // namespace
+
 // reconFDK (fdk_axial.cuh:69-89): every view resident when the estimate is
 // below 75 % of device memory, otherwise views stream through in chunks.
 // Both paths add each voxel's views in scan order, so they give the same
-// volume bit for bit (the property fdk_axial_tests.cu:68-73 asserts).
+// volume bit for bit (the property fdk_axial_tests.cu:68-73 asserts).  A
+// volume that does not fit beside a chunk's buffers (the reference has no
+// path for it) is reconstructed in z slabs, with the same sums again.
 extern "C" gct_status gct_reconstruct(const gct_ct_geom *geom, const float *proj, const gct_image_geom *img, float *vol,
                                       const gct_recon_opts *opts)
 {
@@ -898,72 +1061,46 @@ extern "C" gct_status gct_reconstruct(const gct_ct_geom *geom, const float *proj
         const gct_recon_opts o = opts_or_default(opts);
         validate_geom(*geom);
         validate_image(*geom, *img);
+        if (o.mem_mode < GCT_MEM_AUTO || o.mem_mode > GCT_MEM_SLAB) fail(GCT_ERR_INVALID_ARGUMENT, "unknown mem_mode");
+        if (o.z_slab % 32u) fail(GCT_ERR_INVALID_ARGUMENT, "z_slab must be a multiple of 32 slices");
         DeviceGuard dg(o.device);
         size_t free_b = 0, total_b = 0;
         hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
         const uint64_t nv = is_1d(geom->kind) ? 1u : geom->det_col.num_samples;
         const uint64_t view_bytes = nv * geom->det_row.num_samples * sizeof(float);
         const uint64_t need = uint64_t(geom->num_proj) * view_bytes + volume_bytes(*img) + plan_device_bytes(*geom, *img, o, 0);
-        bool low = (o.mem_mode == GCT_MEM_LOW) ||
-                   (o.mem_mode == GCT_MEM_AUTO && double(need) / double(total_b) >= 0.75);
-        uint32_t chunk = geom->num_proj;
-        if (low) {
-            chunk = o.view_chunk ? o.view_chunk : 0u;
-            if (!chunk) {
-                // chunks of ~1/16 of free memory for staging and filtered views
-                const uint64_t per = view_bytes + (nv + 3u) * ((geom->det_row.num_samples + 18u) & ~15u) * storage_bytes(o.storage);
-                chunk = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(geom->num_proj, (free_b / 16) / per)));
+        int mode = o.mem_mode;
+        if (mode == GCT_MEM_AUTO) {
+            mode = double(need) / double(total_b) < 0.75 ? GCT_MEM_HIGH : GCT_MEM_LOW;
+            if (mode == GCT_MEM_LOW && img->dim == 3) {
+                const uint32_t c = view_chunk(*geom, o, free_b, view_bytes);
+                const uint64_t low = volume_bytes(*img) + plan_device_bytes(*geom, *img, o, c) + 2u * uint64_t(c) * view_bytes;
+                if (double(low) >= 0.9 * double(free_b)) mode = GCT_MEM_SLAB;
             }
-            chunk = std::min(chunk, geom->num_proj);
-            // The filter transforms detector rows in pairs (2k, 2k + 1) counted from the
-            // launch's first view, and a pair's rounding depends on both rows.  Every
-            // chunk must start at an even row, as the whole stack's launch does, so
-            // that both paths pair (and round) alike: with an odd Nv, an even number
-            // of views per chunk.
-            if ((nv & 1u) && (chunk & 1u) && chunk < geom->num_proj) chunk = chunk > 1u ? chunk - 1u : 2u;
         }
+        if (mode == GCT_MEM_SLAB && img->dim == 3) {
+            reconstruct_slabs(*geom, proj, *img, vol, o, free_b, view_bytes);
+            return;
+        }
+        if (mode == GCT_MEM_SLAB) mode = GCT_MEM_LOW;   // a 2-D image is one slice
+        const bool low = mode == GCT_MEM_LOW;
+        const uint32_t chunk = low ? view_chunk(*geom, o, free_b, view_bytes) : geom->num_proj;
         auto plan = make_plan(*geom, *img, o, chunk);
-        hipStream_t s = nullptr;
-        hip_check(hipStreamCreate(&s), "hipStreamCreate");
-        struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamDestroy(s); } } sg{s};
+        Stream s;
         DevBuf dvol(volume_bytes(*img));
         if (!low) {
             DevBuf dproj(size_t(geom->num_proj) * view_bytes);
-            hip_check(hipMemcpyAsync(dproj.p, proj, dproj.bytes, hipMemcpyHostToDevice, s), "copy projections");
-            launch_filter(filter_args(*plan, dproj.as<float>(), 0, 0, plan->V, 0), s);
-            launch_backproject(bp_args(*plan, dvol.as<float>(), 0, plan->V, false, true, 0), s);
-            hip_check(hipMemcpyAsync(vol, dvol.p, dvol.bytes, hipMemcpyDeviceToHost, s), "copy volume");
-            hip_check(hipStreamSynchronize(s), "reconstruct");
+            hip_check(hipMemcpyAsync(dproj.p, proj, dproj.bytes, hipMemcpyHostToDevice, s.s), "copy projections");
+            launch_filter(filter_args(*plan, dproj.as<float>(), 0, 0, plan->V, 0), s.s);
+            launch_backproject(bp_args(*plan, dvol.as<float>(), 0, plan->V, false, true, 0), s.s);
+            hip_check(hipMemcpyAsync(vol, dvol.p, dvol.bytes, hipMemcpyDeviceToHost, s.s), "copy volume");
+            hip_check(hipStreamSynchronize(s.s), "reconstruct");
         } else {
-            // two staging buffers: the copy of chunk i+1 overlaps the work on chunk i
-            DevBuf stage[2] = {DevBuf(size_t(chunk) * view_bytes), DevBuf(size_t(chunk) * view_bytes)};
-            hipEvent_t done[2];
-            hip_check(hipEventCreateWithFlags(&done[0], hipEventDisableTiming), "event");
-            hip_check(hipEventCreateWithFlags(&done[1], hipEventDisableTiming), "event");
-            hipStream_t cs = nullptr;
-            hip_check(hipStreamCreate(&cs), "hipStreamCreate");
-            StreamGuard csg{cs};
-            hipEvent_t copied[2];
-            hip_check(hipEventCreateWithFlags(&copied[0], hipEventDisableTiming), "event");
-            hip_check(hipEventCreateWithFlags(&copied[1], hipEventDisableTiming), "event");
-            bool used[2] = {false, false};
-            uint32_t idx = 0;
-            for (uint32_t first = 0; first < plan->V; first += chunk, idx ^= 1u) {
-                const uint32_t n = std::min(chunk, plan->V - first);
-                if (used[idx]) hip_check(hipStreamWaitEvent(cs, done[idx], 0), "wait");
-                hip_check(hipMemcpyAsync(stage[idx].p, proj + size_t(first) * view_bytes / sizeof(float), size_t(n) * view_bytes,
-                                         hipMemcpyHostToDevice, cs),
-                          "copy projection chunk");
-                hip_check(hipEventRecord(copied[idx], cs), "record");
-                hip_check(hipStreamWaitEvent(s, copied[idx], 0), "wait");
-                launch_filter(filter_args(*plan, stage[idx].as<float>(), first, first, n, first), s);
-                launch_backproject(bp_args(*plan, dvol.as<float>(), first, n, first > 0, first + n >= plan->V, first), s);
-                hip_check(hipEventRecord(done[idx], s), "record");
-                used[idx] = true;
-            }
-            hip_check(hipMemcpyAsync(vol, dvol.p, dvol.bytes, hipMemcpyDeviceToHost, s), "copy volume");
-            hip_check(hipStreamSynchronize(s), "reconstruct");
-            for (int i = 0; i < 2; ++i) { (void)hipEventDestroy(done[i]); (void)hipEventDestroy(copied[i]); }
+            Stager st(size_t(chunk) * view_bytes);
+            stream_views(*plan, proj, view_bytes, chunk, st, dvol.as<float>(), 0, plan->img.dim == 3 ? plan->img.z.num_samples : 1u,
+                         s.s);
+            hip_check(hipMemcpyAsync(vol, dvol.p, dvol.bytes, hipMemcpyDeviceToHost, s.s), "copy volume");
+            hip_check(hipStreamSynchronize(s.s), "reconstruct");
         }
     });
 }

@@ -120,7 +120,13 @@ typedef enum gct_setup_precision {
 typedef enum gct_mem_mode {
     GCT_MEM_AUTO = 0,                /* reconFDK's 75 % of HBM rule (fdk_axial.cuh:85-88) */
     GCT_MEM_HIGH = 1,                /* all views resident */
-    GCT_MEM_LOW = 2                  /* views streamed in chunks */
+    GCT_MEM_LOW = 2,                 /* views streamed in chunks */
+    GCT_MEM_SLAB = 3                 /* volume built in z slabs of z_slab slices (volumes beyond device
+                                      * memory; AUTO picks it for a 3-D image when the volume and a
+                                      * chunk's buffers exceed 90 % of free memory).  The filtered
+                                      * stack is built once and kept when it takes at most half of free
+                                      * memory and view_chunk is 0; otherwise the views stream through
+                                      * again for every slab.  Same volume, bit for bit, as HIGH/LOW */
 } gct_mem_mode;
 
 /* Backprojection kernel family. */
@@ -151,6 +157,7 @@ typedef struct gct_recon_opts {
     uint32_t view_block;             /* views staged per block */
     uint32_t lds_rows;               /* detector rows staged per view */
     uint32_t lds_pairs;              /* (value, difference) pairs staged per row */
+    uint32_t z_slab;                 /* SLAB mode slices per slab, a multiple of 32; 0 = derived */
 } gct_recon_opts;
 
 /* ---- library ----------------------------------------------------------- */
@@ -316,7 +323,7 @@ gct_status gct_batch_execute(gct_batch *batch, const float *const *proj_dev, flo
  * forbid hipMalloc: gct_batch_create + gct_batch_execute(use_graph = 0) +
  * gct_batch_destroy in one call that allocates nothing (no hipMalloc,
  * hipFree, host staging or stream/event creation).  The work table and work
- * counters live in a fixed array of the library on each device (144 KiB:
+ * counters live in a fixed array of the library on each device (about 160 KiB:
  * 256 B of counters and GCT_NOALLOC_MAX_SCANS table entries), held until the
  * process ends; the table reaches it through kernel arguments.  plans[i]
  * reconstructs proj_dev[i] into vol_dev[i], as the plan's own execute would,

@@ -4,6 +4,7 @@ Bar: every voxel within the reference comparator's abs-or-relative 1e-5
 (src/cpp/numeric/comparator.h:48-59) of the oracle's FP32 restatement, for all
 seven geometries, full and short scans, both rotation directions.
 """
+import dataclasses
 import math
 
 import numpy as np
@@ -212,6 +213,46 @@ def test_one_shot_high_low_memory_identical(torch_dev, mode):
     ref, _ = run_plan(torch_dev, wl, proj)
     got = g.reconstruct(wl.scan, proj, wl.img, mem_mode=mode, view_chunk=25)
     assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("bp_kernel", [g.BP_LDS, g.BP_DIRECT], ids=["lds", "direct"])
+@pytest.mark.parametrize("z_slab", [32, 64])
+@pytest.mark.parametrize("view_chunk", [0, 17], ids=["resident", "streamed"])
+@pytest.mark.parametrize("kind", [g.PARALLEL_2D, g.CONE_PARALLEL, g.CYLINDRICAL, g.FLAT],
+                         ids=lambda k: g.KIND_NAMES[k])
+def test_z_slab_reconstruction_identical(torch_dev, kind, view_chunk, z_slab, bp_kernel):
+    """Volumes beyond device memory are built in z slabs (GCT_MEM_SLAB): 80
+    slices in slabs of 32 (32 + 32 + 16) or 64 (64 + 16), with the filtered
+    stack kept resident or the views streamed again per slab in chunks of 17.
+    Every tile and its staged detector box is the whole volume's, so the
+    result equals the high-memory path bit for bit, and the oracle at 1e-5."""
+    base = W.small_case(kind, n=48, views=60, nv=64)
+    img = g.Image(48, 40, 0.5, 0.5, 80, 0.4, mid=(0.3, -0.2, 0.1))
+    wl = dataclasses.replace(base, img=img)
+    proj = wl.projections()
+    high = g.reconstruct(wl.scan, proj, img, mem_mode=g.MEM_HIGH, bp_kernel=bp_kernel)
+    slab = g.reconstruct(wl.scan, proj, img, mem_mode=g.MEM_SLAB, z_slab=z_slab, view_chunk=view_chunk,
+                         bp_kernel=bp_kernel)
+    assert np.array_equal(slab, high)
+    if z_slab == 32 and view_chunk == 0 and bp_kernel == g.BP_LDS:
+        want = O.fdk(wl.scan, img, proj)
+        _report(f"z slab {g.KIND_NAMES[kind]}", slab, want)
+        assert O.are_equal(slab, want, EPS).all()
+
+
+def test_z_slab_options(torch_dev):
+    """z_slab must be a multiple of 32 slices; a 2-D image in slab mode is one
+    slice and reconstructs as the low-memory path does."""
+    wl = W.small_case(g.FLAT, n=32, views=40, nv=48)
+    proj = wl.projections()
+    with pytest.raises(g.GctError):
+        g.reconstruct(wl.scan, proj, wl.img, mem_mode=g.MEM_SLAB, z_slab=48)
+    with pytest.raises(g.GctError):
+        g.reconstruct(wl.scan, proj, wl.img, mem_mode=7)
+    w2 = W.small_case(g.FAN_LINE_1D, n=32, views=40)
+    p2 = w2.projections()
+    high = g.reconstruct(w2.scan, p2, w2.img, mem_mode=g.MEM_HIGH)
+    assert np.array_equal(g.reconstruct(w2.scan, p2, w2.img, mem_mode=g.MEM_SLAB), high)
 
 
 @pytest.mark.parametrize("nv", [80, 40], ids=["rows_inside", "rows_cut"])
