@@ -323,7 +323,8 @@ def run_flat(a, g, W, torch, np, sections):
                           "FP32 peak; the kernel evaluates them strength-reduced along z in fewer instructions, "
                           "so its own limit is valu_issue_frac",
             "valu_issue_frac": valu_issue,
-            "valu_insts_per_update": round(valu_insts / upd, 2) if valu_insts else None,
+            # wave instructions x 64 lanes over the updates (one lane per update)
+            "valu_insts_per_update": round(valu_insts * 64 / upd, 2) if valu_insts else None,
             "traffic": traffic,
             "traffic_profiled_kernel": traffic_kernel,
             "alg_flops_per_update": F_ALG_FLAT,
