@@ -174,6 +174,7 @@ def _declare(L: C.CDLL) -> None:
         "gct_plan_filter": (st, [P, P, C.c_uint32, C.c_uint32, P]),
         "gct_plan_backproject": (st, [P, P, C.c_uint32, C.c_uint32, C.c_int, C.c_int, P]),
         "gct_plan_execute": (st, [P, P, P, P]),
+        "gct_plan_execute_host": (st, [P, P, P, P, P, C.c_uint32, P]),
         "gct_plan_load_filtered": (st, [P, P, P]),
         "gct_plan_store_filtered": (st, [P, P, P]),
         "gct_plan_forward_project": (st, [P, P, P, P]),
@@ -242,6 +243,29 @@ def _dev(t, shape, what: str) -> int:
     if t.numel() != n:
         raise ValueError(f"{what} holds {t.numel()} values; {tuple(shape)} needs {n}")
     return t.data_ptr()
+
+
+def _host(t, shape, what: str) -> int:
+    """Address of a host buffer of float32 values of `shape`: a C-contiguous
+    float32 numpy array or a contiguous float32 torch tensor in host memory
+    (pinned or not) with exactly that many elements."""
+    torch = sys.modules.get("torch")
+    n = int(np.prod(shape))
+    if isinstance(t, np.ndarray):
+        if t.dtype != np.float32 or not t.flags["C_CONTIGUOUS"]:
+            raise ValueError(f"{what} must be a C-contiguous float32 array")
+        if t.size != n:
+            raise ValueError(f"{what} holds {t.size} values; {tuple(shape)} needs {n}")
+        return t.ctypes.data
+    if torch is not None and isinstance(t, torch.Tensor):
+        if t.device.type != "cpu":
+            raise ValueError(f"{what} must be in host memory, not on {t.device}")
+        if t.dtype != torch.float32 or not t.is_contiguous():
+            raise ValueError(f"{what} must be a contiguous float32 tensor")
+        if t.numel() != n:
+            raise ValueError(f"{what} holds {t.numel()} values; {tuple(shape)} needs {n}")
+        return t.data_ptr()
+    raise TypeError(f"{what} must be a numpy array or a host tensor, not {type(t).__name__}")
 
 
 # ---- geometry --------------------------------------------------------------
@@ -394,12 +418,14 @@ def required_memory(scan: Scan, img: Image, **kw) -> int:
     return n.value
 
 
-def reconstruct(scan: Scan, proj: np.ndarray, img: Image, **kw) -> np.ndarray:
-    """One-shot host-to-host reconstruction (Recon::CUDA::reconstruct)."""
+def reconstruct(scan: Scan, proj: np.ndarray, img: Image, out: np.ndarray | None = None, **kw) -> np.ndarray:
+    """One-shot host-to-host reconstruction (Recon::CUDA::reconstruct); the
+    volume goes to `out` when given (a float32 array of the image's shape)."""
     proj = np.ascontiguousarray(proj, dtype=np.float32)
     if proj.size != scan.num_proj * scan.nv * scan.nu:
         raise ValueError("projection buffer does not match the scan")
-    vol = np.zeros(img.shape, np.float32)
+    vol = np.zeros(img.shape, np.float32) if out is None else out
+    _host(vol, img.shape, "out")
     o = opts(**kw)
     check(lib().gct_reconstruct(C.byref(scan.c()), proj.ctypes.data, C.byref(img.c()), vol.ctypes.data, C.byref(o)))
     return vol
@@ -446,6 +472,14 @@ class Plan:
     def execute(self, proj, vol, stream=None):
         p, v = _dev(proj, self.scan.proj_shape, "proj"), _dev(vol, self.img.shape, "vol")
         check(lib().gct_plan_execute(self.h, p, v, _stream(stream)))
+
+    def execute_host(self, proj_host, proj_dev, vol_dev, vol_host, chunk_views: int = 0, stream=None):
+        """Host projections in, host volume out, the upload pipelined with the
+        filter and backprojection (gct_plan_execute_host); proj_dev and
+        vol_dev are the device buffers it works in."""
+        ph, vh = _host(proj_host, self.scan.proj_shape, "proj_host"), _host(vol_host, self.img.shape, "vol_host")
+        p, v = _dev(proj_dev, self.scan.proj_shape, "proj_dev"), _dev(vol_dev, self.img.shape, "vol_dev")
+        check(lib().gct_plan_execute_host(self.h, ph, p, v, vh, int(chunk_views), _stream(stream)))
 
     def load_filtered(self, filt, stream=None):
         f = _dev(filt, self.scan.proj_shape, "filt")

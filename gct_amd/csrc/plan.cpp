@@ -889,9 +889,12 @@ extern "C" gct_status gct_required_memory(const gct_ct_geom *geom, const gct_ima
 
 namespace {
 
+// Streams of the library's own work.  Non-blocking: a blocking stream waits
+// for, and holds up, the null stream, so copies on it could not overlap a
+// caller's kernels on the null stream.
 struct Stream {
     hipStream_t s = nullptr;
-    Stream() { hip_check(hipStreamCreate(&s), "hipStreamCreate"); }
+    Stream() { hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate"); }
     ~Stream() { (void)hipStreamDestroy(s); }
     Stream(const Stream &) = delete;
     Stream &operator=(const Stream &) = delete;
@@ -1044,7 +1047,62 @@ void reconstruct_slabs(const gct_ct_geom &geom, const float *proj, const gct_ima
     hip_check(hipStreamSynchronize(s.s), "reconstruct slabs");
 }
 
+// Views per chunk of a pipelined host-to-host execution: about an eighth of
+// the scan, even when Nv is odd (the filter's row pairs, as in view_chunk).
+uint32_t pipeline_chunk(const gct_plan &p, uint32_t requested)
+{
+    uint32_t c = requested ? std::min(requested, p.V) : std::max(1u, (p.V + 7u) / 8u);
+    if ((p.nv & 1u) && (c & 1u) && c < p.V) c = c > 1u ? c - 1u : 2u;
+    return c;
+}
+
+// Host projections -> device -> volume -> host, pipelined on `s`: views are
+// uploaded chunk by chunk on a stream of their own, and each chunk is
+// filtered and backprojected (accumulating) as soon as it is on the device,
+// so the upload of chunk i + 1 overlaps the work on chunk i; the volume is
+// copied back after the last chunk.  Each voxel adds its views in scan order,
+// so the volume is the one-launch path's bit for bit.  Enqueued in chunk
+// order: a copy from pageable memory that holds the host thread still lets
+// the previous chunk's kernels run meanwhile.
+void execute_host_pipelined(gct_plan &p, const float *proj_host, float *proj_dev, float *vol_dev, float *vol_host,
+                            uint32_t chunk, hipStream_t s)
+{
+    const size_t view_elems = size_t(p.nv) * p.nu;
+    const uint32_t nz = p.img.dim == 3 ? p.img.z.num_samples : 1u;
+    const size_t vol_bytes = size_t(p.img.x.num_samples) * p.img.y.num_samples * nz * sizeof(float);
+    const uint32_t nchunks = (p.V + chunk - 1u) / chunk;
+    Stream cs;
+    Event ready;
+    std::vector<Event> copied(nchunks);
+    // the uploads overwrite proj_dev: after the work already queued on s
+    hip_check(hipEventRecord(ready.e, s), "record");
+    hip_check(hipStreamWaitEvent(cs.s, ready.e, 0), "wait");
+    for (uint32_t i = 0; i < nchunks; ++i) {
+        const uint32_t first = i * chunk, n = std::min(chunk, p.V - first);
+        hip_check(hipMemcpyAsync(proj_dev + first * view_elems, proj_host + first * view_elems,
+                                 size_t(n) * view_elems * sizeof(float), hipMemcpyHostToDevice, cs.s),
+                  "copy projection chunk");
+        hip_check(hipEventRecord(copied[i].e, cs.s), "record");
+        hip_check(hipStreamWaitEvent(s, copied[i].e, 0), "wait");
+        launch_filter(filter_args(p, proj_dev, 0, first, n, 0), s);
+        launch_backproject(bp_args(p, vol_dev, first, n, first > 0, first + n >= p.V, 0), s);
+    }
+    hip_check(hipMemcpyAsync(vol_host, vol_dev, vol_bytes, hipMemcpyDeviceToHost, s), "copy volume");
+}
+
 }  // namespace
+
+extern "C" gct_status gct_plan_execute_host(gct_plan *p, const float *proj_host, float *proj_dev, float *vol_dev,
+                                            float *vol_host, uint32_t chunk_views, void *stream)
+{
+    return guarded([&] {
+        if (!p || !proj_host || !proj_dev || !vol_dev || !vol_host) fail(GCT_ERR_INVALID_ARGUMENT, "null argument");
+        if (p->filt_views < p->V) fail(GCT_ERR_INVALID_ARGUMENT, "plan holds a partial filtered buffer");
+        DeviceGuard dg(p->device);
+        execute_host_pipelined(*p, proj_host, proj_dev, vol_dev, vol_host, pipeline_chunk(*p, chunk_views),
+                               static_cast<hipStream_t>(stream));
+    });
+}
 
 // reconFDK (fdk_axial.cuh:69-89): every view resident when the estimate is
 // below 75 % of device memory, otherwise views stream through in chunks.
@@ -1089,11 +1147,10 @@ extern "C" gct_status gct_reconstruct(const gct_ct_geom *geom, const float *proj
         Stream s;
         DevBuf dvol(volume_bytes(*img));
         if (!low) {
+            // every view resident; the upload is pipelined with the work
             DevBuf dproj(size_t(geom->num_proj) * view_bytes);
-            hip_check(hipMemcpyAsync(dproj.p, proj, dproj.bytes, hipMemcpyHostToDevice, s.s), "copy projections");
-            launch_filter(filter_args(*plan, dproj.as<float>(), 0, 0, plan->V, 0), s.s);
-            launch_backproject(bp_args(*plan, dvol.as<float>(), 0, plan->V, false, true, 0), s.s);
-            hip_check(hipMemcpyAsync(vol, dvol.p, dvol.bytes, hipMemcpyDeviceToHost, s.s), "copy volume");
+            execute_host_pipelined(*plan, proj, dproj.as<float>(), dvol.as<float>(), vol,
+                                   pipeline_chunk(*plan, o.view_chunk), s.s);
             hip_check(hipStreamSynchronize(s.s), "reconstruct");
         } else {
             Stager st(size_t(chunk) * view_bytes);

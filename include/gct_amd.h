@@ -144,7 +144,8 @@ typedef struct gct_recon_opts {
     int32_t mem_mode;                /* gct_mem_mode, default AUTO */
     int32_t device;                  /* HIP device ordinal, default 0 (fdk_axial.cuh:73) */
     uint32_t z_unroll;               /* BP voxels per thread along z; 0 = tuned */
-    uint32_t view_chunk;             /* LOW mode views per chunk; 0 = derived */
+    uint32_t view_chunk;             /* LOW and SLAB modes: views per chunk; HIGH: views per upload chunk of
+                                        the pipelined copy (gct_plan_execute_host); 0 = derived */
     int32_t bp_kernel;               /* gct_bp_kernel, default AUTO */
     /* LDS backprojection shape (z_unroll, view_block, lds_rows, lds_pairs): one of
      * (16, 2, 40, 40), (16, 2, 32, 40), (16, 2, 40, 48), (16, 3, 28, 34), (16, 4, 24, 28), (32, 3, 48, 34),
@@ -286,6 +287,18 @@ gct_status gct_plan_backproject(gct_plan *plan, float *vol_dev, uint32_t first_v
 
 /* All stages, all views: proj_dev [V][Nv][Nu] -> vol_dev. */
 gct_status gct_plan_execute(gct_plan *plan, const float *proj_dev, float *vol_dev, void *stream);
+
+/* All stages from host projections to a host volume, as reconFDKHighMem
+ * (fdk_axial.cuh:152-205) with its copies: the views are uploaded in chunks of
+ * chunk_views (0: about an eighth of the scan) on a stream of the library's,
+ * into proj_dev, and each chunk is filtered and backprojected into vol_dev on
+ * `stream` as soon as it is there, so the upload of one chunk overlaps the
+ * work on the previous one; vol_dev is then copied to vol_host.  Same volume
+ * as gct_plan_execute, bit for bit.  Asynchronous on `stream` for pinned host
+ * memory (keep both host buffers alive until it completes); pageable memory
+ * works too, with the copies holding the calling thread. */
+gct_status gct_plan_execute_host(gct_plan *plan, const float *proj_host, float *proj_dev, float *vol_dev,
+                                 float *vol_host, uint32_t chunk_views, void *stream);
 
 /* Copies filtered projections [V][Nv][Nu] (FP32) into / out of the plan's
  * padded buffer, to test the stages separately. */

@@ -215,6 +215,33 @@ def test_one_shot_high_low_memory_identical(torch_dev, mode):
     assert np.array_equal(got, ref)
 
 
+@pytest.mark.parametrize("pinned", [True, False], ids=["pinned", "pageable"])
+@pytest.mark.parametrize("chunk", [0, 7, 1000])
+def test_execute_host_pipelined_identical(torch_dev, pinned, chunk):
+    """Host projections in, host volume out, with the upload of each chunk of
+    views overlapping the filter and backprojection of the previous one
+    (gct_plan_execute_host): chunks of about V / 8, of 7 views (rounded down
+    to 6, since Nv = 61 is odd and the filter pairs rows from a chunk's first
+    view) and of more than V; pinned and pageable host buffers.  The volume is
+    the device path's bit for bit."""
+    torch = torch_dev
+    wl = W.small_case(g.FLAT, views=96, nv=61)
+    proj = wl.projections()
+    ref, plan = run_plan(torch, wl, proj)
+    if pinned:
+        ph = torch.from_numpy(proj).pin_memory()
+        vh = torch.full(wl.img.shape, 5.0, dtype=torch.float32).pin_memory()
+    else:
+        ph, vh = proj, np.full(wl.img.shape, 5.0, np.float32)
+    pd = torch.empty(wl.scan.proj_shape, dtype=torch.float32, device="cuda")
+    vd = torch.full(wl.img.shape, 7.0, dtype=torch.float32, device="cuda")
+    plan.execute_host(ph, pd, vd, vh, chunk)
+    torch.cuda.synchronize()
+    got = vh.numpy() if pinned else vh
+    assert np.array_equal(got, ref)
+    assert torch.equal(pd.cpu(), torch.from_numpy(proj))
+
+
 @pytest.mark.parametrize("bp_kernel", [g.BP_LDS, g.BP_DIRECT], ids=["lds", "direct"])
 @pytest.mark.parametrize("z_slab", [32, 64])
 @pytest.mark.parametrize("view_chunk", [0, 17], ids=["resident", "streamed"])

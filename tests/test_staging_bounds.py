@@ -88,6 +88,20 @@ def test_device_arguments_are_checked_before_any_call():
         g._dev(torch.zeros(2, 3), (2, 3), "vol")
 
 
+def test_host_arguments_are_checked_before_any_call():
+    torch = pytest.importorskip("torch")
+    assert g._host(np.zeros((2, 3), np.float32), (2, 3), "vol") != 0
+    assert g._host(torch.zeros(2, 3), (2, 3), "vol") != 0
+    with pytest.raises(ValueError, match="float32"):
+        g._host(np.zeros((2, 3), np.float64), (2, 3), "vol")
+    with pytest.raises(ValueError, match="float32"):
+        g._host(np.zeros((3, 2), np.float32).T, (2, 3), "vol")
+    with pytest.raises(ValueError, match="needs 6"):
+        g._host(np.zeros(5, np.float32), (2, 3), "vol")
+    with pytest.raises(TypeError):
+        g._host(12345, (2, 3), "vol")
+
+
 def test_batch_argument_lengths_are_checked():
     class FakePlan:
         def __init__(self):
