@@ -37,9 +37,12 @@ joseph    The Joseph forward projector alone (the served requests'
           reprojection) on the largest request of the serving mix and on the
           flat-cone workload's volume and scan: rays per second, device volume
           and projections resident.
+next      The rows beside the core path: the one-shot reconstruct's high,
+          low-memory and z-slab modes, MetaImage I/O, spiral FDK and a
+          projection-matrix (tilted detector) scan.
 
     python bench.py [--steps K] [--warmup W] [--no-cpu-baseline]
-                    [--sections flat,perview,precision,batched,served,joseph]
+                    [--sections flat,perview,c1,precision,batched,served,joseph,next]
 """
 from __future__ import annotations
 
@@ -700,13 +703,106 @@ def run_joseph(a, g, W, torch, np):
     return res
 
 
+def run_next(a, g, W, torch, np):
+    """The rows beside the core path, each timed on a full-size workload:
+    f1 streaming and z slabs (the one-shot host-to-host reconstruct of the
+    flat-cone workload in its high-memory, low-memory and slab modes, pinned
+    host buffers, volumes compared bit for bit), f2 file I/O (the same volume
+    written and read back as MetaImage), f3 spiral FDK (a helical
+    cylindrical slab, one-shot from a host array) and f4 projection-matrix
+    geometry (the flat-cone scan with turned and tilted detectors: per-voxel
+    set-up on the direct kernel, device-resident, HIP events)."""
+    import tempfile
+    res = {}
+
+    def wall(fn, reps=3):
+        fn()
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t) * 1e3)
+        return statistics.median(ts)
+
+    # f1: high / low / slab memory paths, host to host
+    wl = W.flat_cone_256(noise=False)
+    ph = torch.empty(wl.scan.proj_shape, dtype=torch.float32).pin_memory()
+    ph.copy_(wl.projections_device().cpu())
+    phn = ph.numpy()
+    outs = {}
+    modes = {"high": dict(mem_mode=g.MEM_HIGH), "low_chunks_64": dict(mem_mode=g.MEM_LOW, view_chunk=64),
+             "slab_64_resident": dict(mem_mode=g.MEM_SLAB, z_slab=64),
+             "slab_64_streamed": dict(mem_mode=g.MEM_SLAB, z_slab=64, view_chunk=64)}
+    f1 = {"workload": wl.name, "what": "gct_reconstruct host to host from pinned buffers, plan set-up and "
+                                       "device allocations included; median of 3 after one call"}
+    for name, kw in modes.items():
+        vh = torch.empty(wl.img.shape, dtype=torch.float32).pin_memory().numpy()
+        ms = wall(lambda: g.reconstruct(wl.scan, phn, wl.img, window=wl.window, out=vh, **kw))
+        outs[name] = vh
+        f1[name] = {"ms": round(ms, 2), "gups": round(wl.updates / ms / 1e6, 1)}
+    f1["identical"] = all(np.array_equal(outs[k], outs["high"]) for k in outs)
+    res["f1_streaming"] = f1
+
+    # f2: MetaImage write + read of that volume
+    vol = outs["high"]
+    with tempfile.TemporaryDirectory() as d:
+        fn = os.path.join(d, "vol.mha")
+        t = time.perf_counter()
+        g.write_meta(fn, vol)
+        t_w = time.perf_counter() - t
+        t = time.perf_counter()
+        back = g.read_array(fn)
+        t_r = time.perf_counter() - t
+    res["f2_io"] = {"what": f"MetaImage .mha of the {list(vol.shape)} float32 volume", "bytes": int(vol.nbytes),
+                    "write_gbs": round(vol.nbytes / t_w / 1e9, 2), "read_gbs": round(vol.nbytes / t_r / 1e9, 2),
+                    "round_trip_identical": bool(np.array_equal(back.reshape(vol.shape), vol))}
+    del ph, phn, outs, vol, back
+
+    # f3: spiral FDK, one shot
+    sw = W.spiral_cylindrical_slab()
+    sp = sw.projections_device().cpu().numpy()
+    win = g.spiral_windows(sw.scan, sw.img)
+    upd = int(sw.img.nx) * sw.img.ny * int(np.sum(win[:, 1] - win[:, 0]))
+    sv = np.empty(sw.img.shape, np.float32)
+    ms = wall(lambda: sv.__setitem__(Ellipsis, g.reconstruct_spiral(sw.scan, sp, sw.img)))
+    res["f3_spiral"] = {"workload": f"{sw.name}: {sw.scan.num_proj} views of {sw.scan.nv} x {sw.scan.nu}, "
+                                    f"volume {list(sw.img.shape)}, one rotation of views per slice",
+                        "updates": upd, "ms": round(ms, 2), "gups": round(upd / ms / 1e6, 1),
+                        "what": "gct_reconstruct_spiral from a pageable host array (upload of "
+                                f"{sp.nbytes / 1e6:.0f} MB, plan set-up and allocations included)"}
+    del sp, sv
+
+    # f4: projection-matrix (non-circular) geometry, device resident
+    tw = W.tilted_flat_cone_256()
+    plan = g.Plan(tw.scan, tw.img, window=tw.window)
+    info = plan.info()
+    p = tw.projections_device()
+    v = torch.empty(tw.img.shape, dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+    plan.filter(p, stream=stream)
+    plan.backproject(v, stream=stream)
+    torch.cuda.synchronize()
+    bp_ms = statistics.median(timed_events(torch, stream, lambda: plan.backproject(v, stream=stream), 5))
+    res["f4_projection_matrices"] = {
+        "workload": "flat_cone_256 with every detector turned 1.5 deg in plane and tilted 2.5 deg about its "
+                    "row axis, views from 3x4 projection matrices",
+        "circular": int(info.circular), "setup_f64": int(info.setup_f64),
+        "roofline": roofline_entry("flat", tw.updates, bp_ms,
+                                   tw.scan.num_proj * tw.scan.nv * tw.scan.nu * 4 + tw.img.nvox * 4)}
+    plan.close()
+    del p, v
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--sections", default="flat,perview,c1,precision,batched,served,joseph")
+    ap.add_argument("--sections", default="flat,perview,c1,precision,batched,served,joseph,next")
     ap.add_argument("--threads", type=int, default=0, help="host threads of the CPU legs (0: host_cores())")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy 5 times (reference kernel for HBM byte counters)")
@@ -758,6 +854,10 @@ def main():
     if "joseph" in sections:
         print("bench: joseph section", file=sys.stderr, flush=True)
         out["joseph"] = run_joseph(a, g, W, torch, np)
+        torch.cuda.empty_cache()
+    if "next" in sections:
+        print("bench: next-rows section", file=sys.stderr, flush=True)
+        out["next_rows"] = run_next(a, g, W, torch, np)
         torch.cuda.empty_cache()
     if not a.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)

@@ -276,6 +276,57 @@ constexpr bool column_invariant()
     return CIRC || KIND == GCT_PARALLEL_2D || KIND == GCT_CONE_PARALLEL;
 }
 
+// Flat panels whose row axis is not along z (views from projection matrices:
+// tilted or turned detectors): u, v and w all move along a voxel column.  Per
+// view and 16-slice z block, the numerators at the block's first slice zb are
+// formed once (in FP64 with the FP64 set-up, then rounded); slice zb + kk takes
+// fma(kk, step, base) of each, its reciprocal, column, row and weight -- the
+// same evaluation in the direct and the LDS kernels, so they agree bit for
+// bit, and a result depends on z alone, not on how z is tiled.
+struct NcBase {
+    float nw, nu, nv;   // numerators at the block's first slice
+    float dw, du, dv;   // their steps per slice
+    float k0, k1;       // padded detector centre (column, row)
+};
+
+template <bool F64>
+__device__ inline NcBase nc_base(const BpLaunch &a, uint32_t v, float fx, float fy, float zb)
+{
+    NcBase b;
+    if constexpr (F64) {
+        const BpRec64 &r = a.rec64[v];
+        const double X = fx, Y = fy, Z = zb;
+        b.nw = static_cast<float>(fma(r.w[3], Z, fma(r.w[2], Y, fma(r.w[1], X, r.w[0]))));
+        b.nu = static_cast<float>(fma(r.u[3], Z, fma(r.u[2], Y, fma(r.u[1], X, r.u[0]))));
+        b.nv = static_cast<float>(fma(r.v[3], Z, fma(r.v[2], Y, fma(r.v[1], X, r.v[0]))));
+    } else {
+        const BpRec &r = a.rec[v];
+        b.nw = fmaf(r.w[3], zb, fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0])));
+        b.nu = fmaf(r.u[3], zb, fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0])));
+        b.nv = fmaf(r.v[3], zb, fmaf(r.v[2], fy, fmaf(r.v[1], fx, r.v[0])));
+    }
+    const BpRec &r = a.rec[v];
+    b.dw = r.w[3];
+    b.du = r.u[3];
+    b.dv = r.v[3];
+    b.k0 = r.k[0];
+    b.k1 = r.k[1];
+    return b;
+}
+
+// Column, row (padded, unclamped) and weight of slice zb + kk; kk == 0 takes
+// the bases as they are (fma(0, step, base) is base).
+__device__ inline void nc_map(const NcBase &b, float kk, bool first, float &col, float &row, float &w)
+{
+    const float nw = first ? b.nw : fmaf(kk, b.dw, b.nw);
+    const float nu = first ? b.nu : fmaf(kk, b.du, b.nu);
+    const float nv = first ? b.nv : fmaf(kk, b.dv, b.nv);
+    const float rd = __builtin_amdgcn_rcpf(nw);   // v_rcp_f32 (1 ulp), as the circular set-up
+    col = fmaf(nu, rd, b.k0);
+    row = fmaf(nv, rd, b.k1);
+    w = rd * rd;
+}
+
 // One 16 x 16 x ZU voxel tile: each thread owns one (x, y) column of ZU
 // voxels and accumulates over the views in order, so every voxel sums its
 // views in the reference's order (fdk_axial.cuh:189-193) starting from 0 (or
@@ -322,6 +373,16 @@ __device__ inline void bp_tile3d(const BpLaunch &a, uint32_t bx, uint32_t by, ui
                     const float nvk = fmaf(kk, s.dnv, nvb);
                     w = __builtin_amdgcn_rcpf(fmaf(nvk, nvk, s.rho2));
                 }
+                acc[k] = fmaf(val, w, acc[k]);
+            }
+        } else if constexpr (KIND == GCT_FLAT) {
+            const NcBase nb = nc_base<F64>(a, v, fx, fy, fzb);
+#pragma unroll
+            for (int k = 0; k < ZU; ++k) {
+                float col, row, w;
+                nc_map(nb, fko + static_cast<float>(k), (iz0 & (kZBlock - 1u)) + static_cast<uint32_t>(k) == 0u, col,
+                       row, w);
+                const float val = bilinear<S>(p, a.pitch, col_sample(col, nup), row, nvp);
                 acc[k] = fmaf(val, w, acc[k]);
             }
         } else {
@@ -466,7 +527,9 @@ struct alignas(16) Box {
 // PADZ: the tile's last z corner is taken past the volume's last slice, so
 // the box also holds the rows of the slices a partial z tile computes and
 // discards (needed where the kernel does not clamp rows).
-template <int KIND, int ZU, bool PADZ = false>
+// NC: a flat panel whose u and w move along z (nc_base); each corner is
+// projected at its own z.
+template <int KIND, int ZU, bool PADZ = false, bool NC = false>
 __device__ inline void tile_boxes(const BpLaunch &a, uint32_t bx, uint32_t by, uint32_t bz, uint32_t vb0,
                                   uint32_t nvb, int TR, int TP, Box *boxes, uint32_t tid)
 {
@@ -476,11 +539,20 @@ __device__ inline void tile_boxes(const BpLaunch &a, uint32_t bx, uint32_t by, u
     const uint32_t cy = min(by * 16u + ((corner & 2u) ? 15u : 0u), a.ny - 1u);
     const uint32_t czt = a.z0 + bz * ZU + ((corner & 4u) ? static_cast<uint32_t>(ZU - 1) : 0u);
     const uint32_t cz = PADZ ? czt : min(czt, a.nz - 1u);
-    const float nuf = static_cast<float>(a.nu);
-    const ColSetup s = setup_column<KIND, false, false>(a, vb0 + j, static_cast<float>(cx), static_cast<float>(cy), 0.0f, nuf);
-    const float row = fmaf(static_cast<float>(cz), s.drow, s.row0);
+    float col, row;
+    if constexpr (NC) {
+        float w;
+        nc_map(nc_base<false>(a, vb0 + j, static_cast<float>(cx), static_cast<float>(cy), static_cast<float>(cz)),
+               0.0f, true, col, row, w);
+    } else {
+        const float nuf = static_cast<float>(a.nu);
+        const ColSetup s =
+            setup_column<KIND, false, false>(a, vb0 + j, static_cast<float>(cx), static_cast<float>(cy), 0.0f, nuf);
+        col = s.col;
+        row = fmaf(static_cast<float>(cz), s.drow, s.row0);
+    }
     // the 8 corners of a view are lanes 8j .. 8j + 7, all active
-    const float cmin = min8(s.col), cmax = max8(s.col), rmin = min8(row), rmax = max8(row);
+    const float cmin = min8(col), cmax = max8(col), rmin = min8(row), rmax = max8(row);
     if (corner != 0u) return;
     Box b{};
     const bool finite = cmin > -1.0e6f && cmax < 1.0e6f && rmin > -1.0e6f && rmax < 1.0e6f;
@@ -852,6 +924,144 @@ void bp3d_lds_kernel(BpLaunch a)
     bp_tile3d_lds<KIND, S, F64, ZU, TR, VB, TP, WIN, CLAMP>(a, bx, rest % nby, rest / nby, threadIdx.x, tile, boxes);
 }
 
+// One 16 x 16 x 16 tile of a flat-panel scan whose row axis is not along z
+// (views from projection matrices; NcBase): the staging of bp_tile3d_lds --
+// boxes from the tile's corners, each projected at its own z, staged as
+// (value, difference) pairs -- with column, row and weight evaluated per
+// voxel (nc_map) and both coordinates clamped to the box.  The box covers
+// the tile's taps with a margin, so the clamps bind only where the detector's
+// edge cuts the box, where they reproduce its zero border; per update the
+// taps and arithmetic are those of the direct kernel's non-circular path, so
+// the two agree bit for bit.
+template <int S, bool F64, int TR, int VB, int TP>
+__device__ void bp_tile3d_lds_nc(const BpLaunch &a, uint32_t bx, uint32_t by, uint32_t bz, uint32_t tid,
+                                 float2 *tile, Box *boxes)
+{
+    constexpr int ZU = static_cast<int>(kZBlock);
+    using T = typename Stor<S>::T;
+    const uint32_t ix = bx * 16u + ((tid >> 6) & 1u) * 8u + ((tid >> 4) & 1u) * 4u + (tid & 3u);
+    const uint32_t iy = by * 16u + (tid >> 7) * 8u + ((tid >> 5) & 1u) * 4u + ((tid >> 2) & 3u);
+    const uint32_t iz0 = a.z0 + bz * ZU;   // global slice index, the first of a z block
+    const bool active = ix < a.nx && iy < a.ny;
+    const uint32_t zn = min(static_cast<uint32_t>(ZU), a.z0 + a.zcount - iz0);
+    const size_t slice = static_cast<size_t>(a.nx) * a.ny;
+    float *vp = a.vol + static_cast<size_t>(iz0 - a.z0) * slice + static_cast<size_t>(active ? iy : 0u) * a.nx +
+                (active ? ix : 0u);
+
+    float acc[ZU];
+#pragma unroll
+    for (int k = 0; k < ZU; ++k)
+        acc[k] = (active && a.accumulate && static_cast<uint32_t>(k) < zn) ? vp[k * slice] : 0.0f;
+
+    const float fx = static_cast<float>(ix), fy = static_cast<float>(iy), fz = static_cast<float>(iz0);
+    const float nup = static_cast<float>(a.nu + 1u), nvp = static_cast<float>(a.nv + 1u);
+    const size_t view_elems = static_cast<size_t>(a.rows_pv) * a.pitch;
+    const T *base = static_cast<const T *>(a.filt);
+    const uint32_t vfirst = a.first_view, vend = a.first_view + a.num_views;
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(tid >> 6));
+    const uint32_t lane = tid & 63u;
+    if (vfirst < vend)
+        tile_boxes<GCT_FLAT, ZU, false, true>(a, bx, by, bz, vfirst, min(static_cast<uint32_t>(VB), vend - vfirst), TR,
+                                              TP, boxes, tid);
+    __syncthreads();
+    // fast and general view-block forms as in bp_tile3d_lds
+    uint32_t cur = 0, vb0 = vfirst;
+    auto view_block = [&](auto general) __attribute__((always_inline)) {
+        constexpr bool kGeneral = decltype(general)::value;
+        const uint32_t nvb = min(static_cast<uint32_t>(VB), vend - vb0);
+        const Box *bc = boxes + cur * VB;
+        stage_block<S, TR, VB, TP>(base, view_elems, a.pitch, vb0, nvb, bc, tile, wave, lane);
+        __syncthreads();
+        if (active) {
+            for (uint32_t j = 0; j < nvb; ++j) {
+                const uint32_t v = vb0 + j;
+                const NcBase nb = nc_base<F64>(a, v, fx, fy, fz);
+                const Box b = bc[j];
+                if (!kGeneral || b.ncols != 0) {
+                    // byte offset of pair (r, c): 8 (ebase + c) + 8 TP r, modulo 2^32
+                    const uint32_t eb = 8u * static_cast<uint32_t>(b.ebase);
+                    constexpr int G = 2;
+#pragma unroll
+                    for (int k0 = 0; k0 < ZU; k0 += G) {
+                        float2 tp[G], bt[G];
+                        float alpha[G], beta[G], w[G];
+#pragma unroll
+                        for (int i = 0; i < G; ++i) {
+                            const int k = k0 + i;
+                            float col, row;
+                            nc_map(nb, static_cast<float>(k), k == 0, col, row, w[i]);
+                            const float cc = __builtin_amdgcn_fmed3f(col, b.c0, b.c1);
+                            const float rc = __builtin_amdgcn_fmed3f(row, b.r0, b.r1);
+                            alpha[i] = __builtin_amdgcn_fractf(cc);
+                            beta[i] = __builtin_amdgcn_fractf(rc);
+                            const uint32_t off = __umul24(static_cast<uint32_t>(rc), 8u * TP) + eb +
+                                                 8u * static_cast<uint32_t>(cc);
+                            const float2 *q = reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(tile) + off);
+                            tp[i] = q[0];
+                            bt[i] = q[TP];
+                        }
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int i = 0; i < G; ++i) {
+                            const float top = fmaf(alpha[i], tp[i].y, tp[i].x);
+                            const float bot = fmaf(alpha[i], bt[i].y, bt[i].x);
+                            const float val = fmaf(beta[i], bot - top, top);
+                            acc[k0 + i] = fmaf(val, w[i], acc[k0 + i]);
+                        }
+                    }
+                } else {
+                    const T *p = base + static_cast<size_t>(v) * view_elems;
+#pragma unroll
+                    for (int k = 0; k < ZU; ++k) {
+                        float col, row, w;
+                        nc_map(nb, static_cast<float>(k), k == 0, col, row, w);
+                        const float val = bilinear<S>(p, a.pitch, col_sample(col, nup), row, nvp);
+                        acc[k] = fmaf(val, w, acc[k]);
+                    }
+                }
+            }
+        }
+        const uint32_t vn = vb0 + VB;
+        if (vn < vend)
+            tile_boxes<GCT_FLAT, ZU, false, true>(a, bx, by, bz, vn, min(static_cast<uint32_t>(VB), vend - vn), TR, TP,
+                                                  boxes + (cur ^ 1u) * VB, tid);
+        __syncthreads();
+        vb0 = vn;
+        cur ^= 1u;
+    };
+    for (; vb0 < vend;) {
+        const uint32_t nvb = min(static_cast<uint32_t>(VB), vend - vb0);
+        const Box *bc = boxes + cur * VB;
+        bool fits = true;
+#pragma unroll
+        for (int j = 0; j < VB; ++j)
+            fits = fits && (static_cast<uint32_t>(j) >= nvb || bc[j].ncols != 0);
+        if (!__builtin_amdgcn_readfirstlane(fits)) break;
+        view_block(std::false_type{});
+    }
+    for (; vb0 < vend;) view_block(std::true_type{});
+    if (!active) return;
+    const float sc = a.scale;
+#pragma unroll
+    for (int k = 0; k < ZU; ++k)
+        if (static_cast<uint32_t>(k) < zn) vp[k * slice] = a.finish ? acc[k] * sc : acc[k];
+}
+
+// Non-circular flat scans, LDS-staged: the shape (16, 2, 40, 40), at 5
+// workgroups per CU (up to 96 VGPRs: the per-voxel column, row and weight
+// spill at the 6 its LDS allows).
+template <int S, bool F64>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8)))
+void bp3d_lds_nc_kernel(BpLaunch a)
+{
+    __shared__ float2 tile[2 * 40 * 40];
+    __shared__ Box boxes[2 * 2];
+    const uint32_t nbx = (a.nx + 15u) / 16u, nby = (a.ny + 15u) / 16u;
+    const uint32_t t = a.round_robin ? blockIdx.x : xcd_tile(blockIdx.x, gridDim.x);
+    const uint32_t bx = t % nbx, rest = t / nbx;
+    bp_tile3d_lds_nc<S, F64, 40, 2, 40>(a, bx, rest % nby, rest / nby, threadIdx.x, tile, boxes);
+}
+
 // 2-D images: each block of views has its (padded) detector rows staged whole
 // in LDS; a 16 x 16 pixel tile per workgroup.
 template <int KIND, int S>
@@ -1033,6 +1243,11 @@ void launch_bp3d_variant(const BpLaunch &a, hipStream_t s)
         if (lds && a.circular) {
             if (a.setup_f64) launch_bp3d_lds<KIND, S, true>(a, s);
             else launch_bp3d_lds<KIND, S, false>(a, s);
+        } else if (KIND == GCT_FLAT && lds) {
+            const uint32_t nbx = (a.nx + 15u) / 16u, nby = (a.ny + 15u) / 16u;
+            const dim3 g(nbx * nby * ((a.zcount + kZBlock - 1u) / kZBlock));
+            if (a.setup_f64) hipLaunchKernelGGL((bp3d_lds_nc_kernel<S, true>), g, dim3(256), 0, s, a);
+            else hipLaunchKernelGGL((bp3d_lds_nc_kernel<S, false>), g, dim3(256), 0, s, a);
         } else if (a.setup_f64) {
             if (a.circular) launch_bp3d_zu<KIND, S, true, true>(a, s);
             else launch_bp3d_zu<KIND, S, false, true>(a, s);

@@ -560,6 +560,20 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
         p->lds_rows = sh.rows;
         p->lds_pairs = sh.pairs;
     }
+    // Views whose u and w move along z: flat panels use the LDS kernel of that
+    // case, which has the one shape (16, 2, 40, 40); other kinds the direct kernel.
+    if (!p->circular && p->variant == GCT_BP_LDS && img.dim == 3) {
+        if (geom.kind == GCT_FLAT) {
+            const LdsShape &sh = kLdsShapes[0];
+            p->z_unroll = sh.zu;
+            p->view_block = sh.vb;
+            p->lds_rows = sh.rows;
+            p->lds_pairs = sh.pairs;
+        } else {
+            p->variant = GCT_BP_DIRECT;
+            p->z_unroll = 8u;
+        }
+    }
     // Untuned LDS plans deal their tiles round robin unless the volume is at
     // most 4 tiles deep: measured, round robin is 1.6 % faster over the 3-D
     // requests of the serving mix (up to 6 % on flat cubes), contiguous runs
@@ -1240,7 +1254,12 @@ extern "C" gct_status gct_plan_autotune(gct_plan *p, float *vol, uint32_t *best_
         struct Cand { int variant; uint32_t zu, vb, rows, pairs; bool rr; };
         std::vector<Cand> cands;
         for (const bool rr : {false, true})
-            for (const LdsShape &sh : kLdsShapes) cands.push_back({GCT_BP_LDS, sh.zu, sh.vb, sh.rows, sh.pairs, rr});
+            for (int si = 0; si < kNumLdsShapes; ++si) {
+                // non-circular scans: flat panels have the one LDS shape (16, 2, 40, 40), others none
+                if (!p->circular && (p->geom.kind != GCT_FLAT || si != 0)) continue;
+                const LdsShape &sh = kLdsShapes[si];
+                cands.push_back({GCT_BP_LDS, sh.zu, sh.vb, sh.rows, sh.pairs, rr});
+            }
         cands.push_back({GCT_BP_DIRECT, 8u, 0u, 0u, 0u, false});
         cands.push_back({GCT_BP_DIRECT, 16u, 0u, 0u, 0u, false});
         // Two interleaved rounds (the second only for candidates within 10 % of

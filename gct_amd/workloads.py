@@ -184,15 +184,36 @@ def spiral_case(kind: int = CYLINDRICAL, n: int = 48, rotations: float = 3.0, vi
     return Workload(f"spiral_{kind}", scan, img, ph, 0.0, seed)
 
 
-def tilted_flat_case(n: int = 48, views: int = 90, nu: int = 96, nv: int = 80, seed: int = 43) -> Workload:
-    """A flat-panel scan given by projection matrices, as a calibrated C-arm
-    is: the circular views of small_case with every detector turned in its
-    plane by 1.5 deg and tilted about its row axis by 2.5 deg (so the row axis
-    is no longer along z), each view then passed through its 3 x 4 matrix and
-    decomposed back (ProjectionMatrix compose / decompose)."""
-    base = small_case(FLAT, n=n, views=views, nu=nu, nv=nv, seed=seed)
-    sc = base.scan
-    a, b = math.radians(1.5), math.radians(2.5)
+def spiral_cylindrical_slab(seed: int = 78) -> Workload:
+    """Helical MSCT scan at the uneven batch's slab geometry: cylindrical
+    detector 672 channels x 64 rows (0.6 mm at iso), SOD 570, SDD 1040 mm,
+    1160 views per rotation, pitch 38.4 mm per rotation (pitch factor 1),
+    2.5 rotations; a 512 x 512 x 64 volume at 0.977 x 0.977 x 0.6 mm in the
+    middle of the helix, so every slice gets its full one-rotation window.
+    (The slabs' 1152 views per rotation trip the reference's FP32 rotation
+    analysis over 2.5 rotations: its accumulated range comes out above 2 pi
+    and it counts every view as one rotation, which the spiral path rejects.)"""
+    vpr, rotations = 1160, 2.5
+    views = int(vpr * rotations)
+    incr = 2.0 * math.pi / vpr
+    hel = 38.4 / (2.0 * math.pi)
+    scan = circular_scan(CYLINDRICAL, 672, 0.957 / 672, views, incr, nv=64, dv=1.095, det_row_center=335.5,
+                         det_col_center=31.5, src_det_dist=1040.0, src_iso_dist=570.0, hel_pitch=hel)
+    zc = -hel * incr * (views - 1) / 2.0   # source z at the middle view
+    img = Image(512, 512, 0.977, 0.977, 64, 0.6, mid=(0.0, 0.0, zc))
+    ph = phantom_random(seed, 12, 3, 250.0, (0.05, 0.35), (-0.02, 0.1))
+    ph[:, 2] = ph[:, 2] * 0.1 + zc
+    return Workload("spiral_cyl_slab", scan, img, ph, 0.0, seed)
+
+
+def tilt_views(scan: Scan, turn_deg: float = 1.5, tilt_deg: float = 2.5) -> Scan:
+    """The flat-panel views of `scan` with every detector turned in its plane
+    by turn_deg and tilted about its row axis by tilt_deg (so the row axis is
+    no longer along z), each view then passed through its 3 x 4 projection
+    matrix and decomposed back (ProjectionMatrix compose / decompose), as a
+    calibrated C-arm's geometry arrives."""
+    a, b = math.radians(turn_deg), math.radians(tilt_deg)
+    sc = scan
     out = np.empty_like(sc.views)
     for i, row in enumerate(sc.views):
         v = row.astype(np.float64)
@@ -205,8 +226,22 @@ def tilted_flat_case(n: int = 48, views: int = 90, nu: int = 96, nv: int = 80, s
         t[8:11], t[11:14] = eu1, ev2
         P = projection_matrix(t.astype(np.float32), sc.du, sc.dv)
         out[i] = projection_matrix_view(P, sc.du)
-    scan = Scan(FLAT, sc.nu, sc.du, sc.nv, sc.dv, out)
-    return Workload("tilted_flat", scan, base.img, base.phantom, 0.0, seed)
+    return Scan(FLAT, sc.nu, sc.du, sc.nv, sc.dv, out)
+
+
+def tilted_flat_case(n: int = 48, views: int = 90, nu: int = 96, nv: int = 80, seed: int = 43) -> Workload:
+    """A flat-panel scan given by projection matrices, as a calibrated C-arm
+    is: the circular views of small_case, turned and tilted (tilt_views)."""
+    base = small_case(FLAT, n=n, views=views, nu=nu, nv=nv, seed=seed)
+    return Workload("tilted_flat", tilt_views(base.scan), base.img, base.phantom, 0.0, seed)
+
+
+def tilted_flat_cone_256() -> Workload:
+    """The flat-cone workload's scan turned and tilted (tilt_views): the same
+    volume and phantom, views given by projection matrices."""
+    base = flat_cone_256(noise=False)
+    return Workload("tilted_flat_cone_256", tilt_views(base.scan), base.img, base.phantom, 0.0, base.seed,
+                    base.window)
 
 
 # The reference's end-to-end FDK regression geometries (test/cuda/recon/

@@ -250,7 +250,10 @@ typedef struct gct_plan_info {
     float rot_ang_range;
     int32_t rot_dir;
     int32_t setup_f64;
-    int32_t circular;                /* 1 when every view keeps the row axis along z */
+    int32_t circular;                /* 1 when every view keeps the row axis along z.  0 (e.g. views from
+                                        projection matrices with a tilted detector): flat panels backproject
+                                        with the LDS kernel of that case, shape (16, 2, 40, 40), or the
+                                        direct kernel; other kinds with the direct kernel */
     int32_t storage;
     int32_t bp_kernel;               /* gct_bp_kernel in use */
     uint32_t view_block;             /* LDS BP shape in use (0 for the direct kernel): views per block, */

@@ -186,22 +186,34 @@ def test_flat_cone_256_sampled_slices(torch_dev, window):
         assert O.are_equal(got[z], want, EPS).all()
 
 
+@pytest.mark.parametrize("nv", [80, 40], ids=["rows_inside", "rows_cut"])
 @pytest.mark.parametrize("setup", [g.SETUP_F32, g.SETUP_F64], ids=["f32", "f64"])
-def test_projection_matrix_geometry_tilted_detector(torch_dev, setup):
+def test_projection_matrix_geometry_tilted_detector(torch_dev, setup, nv):
     """A flat scan given by projection matrices whose detectors are turned in
     plane and tilted about their row axis (gct_amd.workloads.tilted_flat_case):
     the rows no longer run along z, the plan reports a non-circular geometry
-    and backprojects with per-voxel set-up; high- and low-memory paths match
-    the oracle's FP32 restatement at 1e-5 and each other bit for bit."""
-    wl = W.tilted_flat_case()
+    and backprojects with per-voxel column, row and weight.  The LDS kernel of
+    that case and the direct kernel agree bit for bit (with 40 detector rows
+    the volume's rows leave the detector: the staged boxes are cut and the
+    clamps give the zero border); high-memory, low-memory and z-slab paths
+    agree bit for bit; all match the oracle's FP32 restatement at 1e-5."""
+    wl = W.tilted_flat_case(nv=nv)
     proj = wl.projections()
     got, plan = run_plan(torch_dev, wl, proj, setup_precision=setup)
-    assert plan.info().circular == 0 and plan.info().setup_f64 == (setup == g.SETUP_F64)
+    info = plan.info()
+    assert info.circular == 0 and info.setup_f64 == (setup == g.SETUP_F64)
+    assert info.bp_kernel == g.BP_LDS and (info.z_unroll, info.view_block, info.lds_rows, info.lds_pairs) == (16, 2, 40, 40)
     want = O.fdk(wl.scan, wl.img, proj)
-    _report(f"tilted flat setup {setup}", got, want)
+    _report(f"tilted flat setup {setup} nv {nv}", got, want)
     assert O.are_equal(got, want, EPS).all()
+    direct, dplan = run_plan(torch_dev, wl, proj, setup_precision=setup, bp_kernel=g.BP_DIRECT)
+    assert dplan.info().bp_kernel == g.BP_DIRECT
+    assert np.array_equal(direct, got)
     low = g.reconstruct(wl.scan, proj, wl.img, mem_mode=g.MEM_LOW, view_chunk=17, setup_precision=setup)
     assert np.array_equal(low, got)
+    img = g.Image(48, 48, 0.5, 0.5, 80, 0.3, mid=(0.3, -0.2, 0.1))
+    slab = g.reconstruct(wl.scan, proj, img, mem_mode=g.MEM_SLAB, z_slab=32, setup_precision=setup)
+    assert np.array_equal(slab, g.reconstruct(wl.scan, proj, img, mem_mode=g.MEM_HIGH, setup_precision=setup))
 
 
 @pytest.mark.parametrize("mode", [g.MEM_HIGH, g.MEM_LOW])
