@@ -90,7 +90,8 @@ class PlanInfo(C.Structure):
                 ("updates", C.c_uint64), ("rot_ang_incr", C.c_float), ("rot_ang_range", C.c_float),
                 ("rot_dir", C.c_int32), ("setup_f64", C.c_int32), ("circular", C.c_int32), ("storage", C.c_int32),
                 ("bp_kernel", C.c_int32), ("view_block", C.c_uint32), ("lds_rows", C.c_uint32),
-                ("lds_pairs", C.c_uint32), ("rows_inside", C.c_int32), ("round_robin", C.c_int32)]
+                ("lds_pairs", C.c_uint32), ("rows_inside", C.c_int32), ("round_robin", C.c_int32),
+                ("narrow_fan", C.c_int32)]
 
 
 class ServerOpts(C.Structure):

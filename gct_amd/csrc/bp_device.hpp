@@ -70,6 +70,34 @@ __device__ inline float fan_atan2(float y, float x)
     return copysignf(r, y);
 }
 
+// Fan angle atan(nu / nw) for scans whose plan showed (BpLaunch::narrow_fan)
+// that every voxel lies in front of the source in every view (nw > 0) and
+// that the detector's padded columns stay within +-0.6 rad less half a column:
+// no octant reduction.  t = nu / nw is clamped to +-tan(0.6), where atan(t) is
+// t * P(t^2) with P of degree 6 fitted on [0, tan 0.6] (atan_fit.py: within
+// 3.3e-9 of atan in FP64, 7.9e-8 evaluated here); a voxel beyond 0.6 rad takes
+// the clamped angle, whose column lies outside the detector, so it reads the
+// zero padding as with the exact angle.
+constexpr float kAtanNarrowT = 6.841368079e-01f;   // tan(0.6) in FP32
+constexpr float kAtanNarrowCoef[7] = {9.999999404e-01f, -3.333281577e-01f, 1.998677999e-01f, -1.413784027e-01f,
+                                      1.024508923e-01f, -6.219936907e-02f, 2.111230977e-02f};
+
+__device__ inline float fan_atan_narrow(float nu, float nw)
+{
+    const float t = __builtin_amdgcn_fmed3f(nu * __builtin_amdgcn_rcpf(nw), -kAtanNarrowT, kAtanNarrowT);
+    const float u = t * t;
+    float p = kAtanNarrowCoef[6];
+#pragma unroll
+    for (int j = 5; j >= 0; --j) p = fmaf(p, u, kAtanNarrowCoef[j]);
+    return t * p;
+}
+
+// The arc detectors' fan angle of (nu, nw) in the FP32 forms.
+__device__ inline float fan_angle(bool narrow, float nu, float nw)
+{
+    return narrow ? fan_atan_narrow(nu, nw) : fan_atan2(nu, nw);
+}
+
 struct Sample {
     uint32_t colbase;   // padded column index of the left tap
     float alpha;        // horizontal fraction
@@ -93,6 +121,17 @@ __device__ inline float below(float x)
     return __int_as_float(__float_as_int(x) - 1);
 }
 
+// The taps are addressed by a 32-bit byte offset from a wave-uniform row
+// pointer, and the right neighbour by pointer arithmetic on that address, so a
+// tap pair is one VGPR offset and two loads with immediate offsets off an SGPR
+// base (a view's padded buffer is far below 4 GiB).
+template <int S>
+__device__ inline const typename Stor<S>::T *tap(const typename Stor<S>::T *p, uint32_t elem)
+{
+    return reinterpret_cast<const typename Stor<S>::T *>(reinterpret_cast<const char *>(p) +
+                                                          elem * static_cast<uint32_t>(sizeof(typename Stor<S>::T)));
+}
+
 // `rowp` is the padded row coordinate (row + 1), clamped like a column.
 template <int S>
 __device__ inline float bilinear(const typename Stor<S>::T *__restrict__ p, uint32_t pitch, Sample cs, float rowp,
@@ -100,9 +139,9 @@ __device__ inline float bilinear(const typename Stor<S>::T *__restrict__ p, uint
 {
     const float r = __builtin_amdgcn_fmed3f(rowp, 0.0f, nvp);
     const float beta = __builtin_amdgcn_fractf(r);
-    const uint32_t off = static_cast<uint32_t>(r) * pitch + cs.colbase;
-    const float t0 = Stor<S>::ld(p, off), t1 = Stor<S>::ld(p, off + 1u);
-    const float b0 = Stor<S>::ld(p, off + pitch), b1 = Stor<S>::ld(p, off + pitch + 1u);
+    const auto *q = tap<S>(p, static_cast<uint32_t>(r) * pitch + cs.colbase);
+    const float t0 = Stor<S>::ld(q, 0), t1 = Stor<S>::ld(q, 1);
+    const float b0 = Stor<S>::ld(q + pitch, 0), b1 = Stor<S>::ld(q + pitch, 1);
     const float top = fmaf(cs.alpha, t1 - t0, t0);
     const float bot = fmaf(cs.alpha, b1 - b0, b0);
     return fmaf(beta, bot - top, top);
@@ -111,8 +150,8 @@ __device__ inline float bilinear(const typename Stor<S>::T *__restrict__ p, uint
 template <int S>
 __device__ inline float linear(const typename Stor<S>::T *__restrict__ p, uint32_t pitch, Sample cs)
 {
-    const uint32_t off = pitch + cs.colbase;
-    const float t0 = Stor<S>::ld(p, off), t1 = Stor<S>::ld(p, off + 1u);
+    const auto *q = tap<S>(p + pitch, cs.colbase);
+    const float t0 = Stor<S>::ld(q, 0), t1 = Stor<S>::ld(q, 1);
     return fmaf(cs.alpha, t1 - t0, t0);
 }
 
@@ -221,7 +260,7 @@ __device__ inline ColSetup setup_column(const BpLaunch &a, uint32_t v, float fx,
             }
             const float rho2 = fmaf(nu, nu, nw * nw);
             const float rr = __builtin_amdgcn_rsqf(rho2);   // v_rsq_f32 (1 ulp)
-            s.col = fmaf(fan_atan2(nu, nw), a.inv_dgamma, r.k[0]);
+            s.col = fmaf(fan_angle(a.narrow_fan, nu, nw), a.inv_dgamma, r.k[0]);
             s.row0 = fmaf(r.k[2] * nv, rr, r.k[1]);
             s.drow = r.k[2] * r.v[3] * rr;
             s.rho2 = rho2;
@@ -431,7 +470,7 @@ __device__ inline void bp_tile2d(const BpLaunch &a, uint32_t ix, uint32_t iy)
         } else {   // GCT_FAN_ARC_1D
             const float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
             const float nu = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
-            col = fmaf(fan_atan2(nu, nw), a.inv_dgamma, r.k[0]);
+            col = fmaf(fan_angle(a.narrow_fan, nu, nw), a.inv_dgamma, r.k[0]);
             w = __builtin_amdgcn_rcpf(fmaf(nu, nu, nw * nw));   // forms scaled by 1/sqrt(SID)
         }
         const float val = linear<S>(p, a.pitch, col_sample(col, nup));
@@ -1104,7 +1143,7 @@ __global__ __launch_bounds__(256) void bp2d_lds_kernel(BpLaunch a, uint32_t vb)
                 } else {
                     const float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
                     const float nu = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
-                    col = fmaf(fan_atan2(nu, nw), a.inv_dgamma, r.k[0]);
+                    col = fmaf(fan_angle(a.narrow_fan, nu, nw), a.inv_dgamma, r.k[0]);
                     w = __builtin_amdgcn_rcpf(fmaf(nu, nu, nw * nw));   // forms scaled by 1/sqrt(SID)
                 }
                 const Sample cs = col_sample(col, nup);

@@ -302,6 +302,34 @@ bool rows_inside_detector(const gct_plan &p, const std::vector<BpRec64> &rec64)
     return true;
 }
 
+// Whether the arc kinds' FP32 kernels may take the fan angle without octant
+// reduction (fan_atan_narrow): in every view every voxel lies in front of the
+// source (nw > 0: nw is affine over the image, so its minimum is at a corner of
+// the image's footprint, the z term included for a cylindrical detector), and
+// both padded edge columns of the detector, 0 and nu + 1, lie within the
+// clamped angle's reach with half a column to spare -- then a voxel whose angle
+// is clamped projects outside the detector, as its exact angle does.  In FP64
+// from the FP64 records.
+bool narrow_fan_ok(const gct_plan &p, const std::vector<BpRec64> &rec64)
+{
+    const gct_ct_geom &g = p.geom;
+    if ((g.kind != GCT_FAN_ARC_1D && g.kind != GCT_CYLINDRICAL) || p.setup_f64) return false;
+    const double reach = 0.5999999 - 0.5 * double(g.det_row.sampling_width);   // the clamp's angle, less half a column
+    if (!(g.det_row.sampling_width > 0.0f) || !(reach > 0.0)) return false;
+    const double xm = double(p.img.x.num_samples) - 1.0, ym = double(p.img.y.num_samples) - 1.0;
+    const double zm = (p.img.dim == 3) ? double(p.img.z.num_samples) - 1.0 : 0.0;
+    for (const BpRec64 &r : rec64) {
+        for (int c = 0; c < 8; ++c) {
+            const double x = (c & 1) ? xm : 0.0, y = (c & 2) ? ym : 0.0, z = (c & 4) ? zm : 0.0;
+            if (!(r.w[0] + r.w[1] * x + r.w[2] * y + r.w[3] * z > 0.0)) return false;
+        }
+        // padded column = gamma / dgamma + k0 (k0 = centre + 1)
+        const double dg = g.det_row.sampling_width;
+        if (!(r.k[0] * dg < reach && (double(p.nu) + 1.0 - r.k[0]) * dg < reach)) return false;
+    }
+    return true;
+}
+
 // The LDS shape a plan without shape options starts from (the autotuner can
 // still replace it): the first of the shapes, in the order they rank on the
 // flat-cone and C-arm workloads, whose staging area holds the boxes of at
@@ -474,6 +502,7 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
     std::vector<FiltRec> frec;
     build_records(*p, rec, rec64, frec);
     p->rows_inside = rows_inside_detector(*p, rec64);
+    p->narrow_fan = narrow_fan_ok(*p, rec64);
     const bool shape_free = !o.z_unroll && !o.view_block && !o.lds_rows && !o.lds_pairs;
     const int auto_shape = (shape_free && variant == GCT_BP_LDS && p->circular && !is_1d(geom.kind))
                                ? default_lds_shape(*p, rec64)
@@ -660,6 +689,7 @@ BpLaunch bp_args(gct_plan &p, float *vol, uint32_t first, uint32_t n, bool accum
     a.lds_rows = p.lds_rows;
     a.lds_pairs = p.lds_pairs;
     a.rows_inside = p.rows_inside;
+    a.narrow_fan = p.narrow_fan;
     a.round_robin = p.round_robin;
     return a;
 }
@@ -816,6 +846,7 @@ extern "C" gct_status gct_plan_get_info(const gct_plan *p, gct_plan_info *info)
         info->lds_pairs = lds ? kLdsShapes[shape].pairs : 0u;
         info->rows_inside = p->rows_inside;
         info->round_robin = lds && p->round_robin;
+        info->narrow_fan = p->narrow_fan;
     });
 }
 

@@ -83,6 +83,7 @@ struct gct_plan {
     uint32_t filt_views = 0;   // views the filtered buffer holds
     bool circular = true;
     bool rows_inside = false;   // every voxel projects inside the detector rows (the LDS kernel drops its row clamp)
+    bool narrow_fan = false;    // arc kinds: the kernels' fan angle needs no octant reduction (narrow_fan_ok)
     bool setup_f64 = false;
     float inv_dgamma = 0.0f;
     float scale = 1.0f;

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define GCT_AMD_ABI_VERSION 1
+#define GCT_AMD_ABI_VERSION 2
 
 /* Status codes.  The reference throws std::invalid_argument / std::runtime_error
  * (src/cuda/utils/check_error.cu:6-36, src/cpp/geom/ct_geom.h:26-32); the C++
@@ -264,6 +264,9 @@ typedef struct gct_plan_info {
     int32_t round_robin;             /* 1 when the LDS backprojection deals its tiles to the XCDs round
                                         robin instead of in contiguous runs (the autotuner's choice, else
                                         round robin for volumes more than 4 tiles deep) */
+    int32_t narrow_fan;              /* arc and cylindrical detectors, FP32 set-up: 1 when every voxel lies
+                                        in front of the source in every view and the detector spans at most
+                                        +-0.6 rad: the kernels take the fan angle without octant reduction */
 } gct_plan_info;
 
 /* Builds per-view tables (FP64 on the host, stored FP32), the redundancy

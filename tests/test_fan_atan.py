@@ -66,3 +66,61 @@ def test_fit_script_reproduces_header():
     spec.loader.exec_module(mod)
     c = mod.fit().astype(np.float32)
     np.testing.assert_allclose(c, _coefs(), rtol=2.4e-7)   # within an ulp of the header
+
+
+def _narrow():
+    txt = SRC.read_text()
+    m = re.search(r"kAtanNarrowCoef\[7\]\s*=\s*\{([^}]*)\}", txt)
+    t = re.search(r"kAtanNarrowT\s*=\s*([0-9.eE+-]+)f", txt)
+    assert m and t, "narrow arctangent constants not found"
+    return (np.array([float(s.strip().rstrip("f")) for s in m.group(1).split(",")], np.float32),
+            np.float32(float(t.group(1))))
+
+
+def _fan_atan_narrow(nu, nw, c, T):
+    """bp_device.hpp fan_atan_narrow, step for step in float32 (the reciprocal
+    is IEEE here, within an ulp of the kernel's v_rcp_f32)."""
+    nu = nu.astype(np.float32)
+    nw = nw.astype(np.float32)
+    t = np.clip((nu * (np.float32(1) / nw)).astype(np.float32), -T, T)
+    u = (t * t).astype(np.float32)
+    p = np.full_like(t, c[6])
+    for j in range(5, -1, -1):
+        p = (p * u + c[j]).astype(np.float32)
+    return (t * p).astype(np.float32)
+
+
+def test_fan_atan_narrow_matches_arctan_in_range():
+    c, T = _narrow()
+    assert abs(float(T) - np.tan(0.6)) < 1e-7
+    rng = np.random.default_rng(11)
+    gam = rng.uniform(-0.6, 0.6, 400000)
+    rad = rng.uniform(1.0, 1e3, gam.size)
+    nu, nw = (rad * np.sin(gam)).astype(np.float32), (rad * np.cos(gam)).astype(np.float32)
+    got = _fan_atan_narrow(nu, nw, c, T).astype(np.float64)
+    ref = np.arctan2(nu.astype(np.float64), nw.astype(np.float64))
+    # 7.9e-8 from the fit evaluated in FP32, plus the rounding of t = nu / nw
+    # (half an ulp of the quotient and of the product) carried through atan'
+    assert np.abs(got - ref).max() < 1.6e-7
+
+
+def test_fan_atan_narrow_clamps_beyond_its_range():
+    """Angles beyond 0.6 rad take the clamped angle (0.6 within the fit), with
+    the right sign: a voxel outside the fan still lands outside the detector."""
+    c, T = _narrow()
+    gam = np.array([0.61, 0.9, 1.4, -0.61, -1.2], np.float64)
+    nu, nw = np.sin(gam).astype(np.float32), np.cos(gam).astype(np.float32)
+    got = _fan_atan_narrow(nu, nw, c, T)
+    np.testing.assert_allclose(np.abs(got), 0.6, atol=1.2e-7)
+    assert np.all(np.sign(got) == np.sign(gam))
+
+
+def test_fit_script_reproduces_narrow_header():
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("atan_fit", SRC.parent / "atan_fit.py")
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    c, T = _narrow()
+    np.testing.assert_allclose(mod.fit_narrow().astype(np.float32), c, rtol=2.4e-7)
+    assert np.float32(mod.NARROW_T) == T

@@ -23,7 +23,7 @@ def test_library_exports_every_declared_symbol():
     L = g.lib()
     missing = [n for n in sorted(names) if not hasattr(L, n)]
     assert not missing, missing
-    assert L.gct_abi_version() == 1
+    assert L.gct_abi_version() == 2
 
 
 def test_no_device_reports_status_without_crashing():
