@@ -797,6 +797,11 @@ void joseph_lds_kernel(FpLaunch a, uint32_t tiles_c, uint32_t tiles_r)
     const uint32_t base = xm ? (a.nz + 3u) * (a.ny * (a.nx + 3u)) : 0u;
     const float nap = static_cast<float>(na + 1u), nbp = static_cast<float>(a.nz + 1u);
     const float *__restrict__ P = a.pad;
+    // both padded copies, for the staging loads' range check (< 2^30 floats: joseph_pad_elems)
+    const uint32_t pad_bytes =
+        4u * (a.nz + 3u) * (a.ny * (a.nx + 3u) + a.nx * (a.ny + 3u));
+    const __amdgpu_buffer_rsrc_t prs =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(P), 0, static_cast<int>(pad_bytes), 0x00020000);
     const uint32_t brick_lds = static_cast<uint32_t>(reinterpret_cast<uintptr_t>((LdsF *)brick));
     for (int s0 = lo; s0 <= hi; s0 += static_cast<int>(kJB)) {
         const int s1 = s0 + static_cast<int>(kJB) - 1;
@@ -843,24 +848,35 @@ void joseph_lds_kernel(FpLaunch a, uint32_t tiles_c, uint32_t tiles_r)
         const uint32_t np = static_cast<uint32_t>(min(static_cast<int>(kJB), hi - s0 + 1));
         if (fits) {
             // Brick element (p, hb, la) = copy element base + (s0 + p) w + (blo + hb) sb + alo + la:
-            // wave q stages rows q, q + 4, ... of the (plane, row) pairs, one tap per lane.  The
-            // clamped coordinates keep every staged column <= na + 2 and row <= nz + 2, inside the copy.
-            // In two halves: 16 loads in flight per lane, then their stores.
+            // wave q stages (plane, row) pairs q, q + 4, ... -- pair 4 k + q is plane k / 4, row
+            // 4 (k % 4) + q -- one tap per lane, whole rows of 64 lanes and all kJB planes whatever
+            // the box: elements past its columns, rows or planes land in the brick and are never
+            // read (the traced taps stay within columns < W, rows < H and planes < np).  The row's
+            // element index is wave-uniform (scalar); the loads are raw buffer loads with the whole
+            // byte offset in the lane's (range-checked) operand, so an element past the copies
+            // reads 0 instead of faulting.  In two halves: 16 loads in flight per lane, then their
+            // stores, each at an immediate offset from the lane's column.
+            const uint32_t wq = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(wave)));
+            const uint32_t alo_s = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(alo)));
+            const uint32_t blo_s = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(blo)));
+            const uint32_t e0 = base + static_cast<uint32_t>(s0) * w + (blo_s + wq) * sb + alo_s;
+            const uint32_t lane4 = lane * 4u;
+            float *bw = brick + wq * kJWP + lane;
             constexpr uint32_t kRows = kJB * kJHM / 4u, kHalf = kRows / 2u;
 #pragma unroll
             for (uint32_t h = 0; h < 2u; ++h) {
                 float val[kHalf];
 #pragma unroll
                 for (uint32_t k = 0; k < kHalf; ++k) {
-                    const uint32_t row = wave + 4u * (h * kHalf + k), p = row / kJHM, hb = row % kJHM;
-                    const bool on = p < np && hb < H && lane < W;
-                    const uint32_t e = base + (static_cast<uint32_t>(s0) + p) * w + (blo + hb) * sb + alo + lane;
-                    val[k] = on ? P[e] : 0.0f;
+                    const uint32_t kk = h * kHalf + k, p = kk / 4u, hq = kk % 4u;
+                    const uint32_t e = e0 + p * w + 4u * hq * sb;
+                    val[k] = __uint_as_float(
+                        __builtin_amdgcn_raw_buffer_load_b32(prs, static_cast<int>(e * 4u + lane4), 0, 0));
                 }
 #pragma unroll
                 for (uint32_t k = 0; k < kHalf; ++k) {
-                    const uint32_t row = wave + 4u * (h * kHalf + k), p = row / kJHM, hb = row % kJHM;
-                    if (p < np && hb < H && lane < W) brick[(p * kJHM + hb) * kJWP + lane] = val[k];
+                    const uint32_t kk = h * kHalf + k, p = kk / 4u, hq = kk % 4u;
+                    bw[(p * kJHM + 4u * hq) * kJWP] = val[k];
                 }
             }
         }
