@@ -556,11 +556,16 @@ __device__ inline float max8(float x)
 // to [c0, c1] and [r0, r1] (just below the last staged column and row, so
 // both taps of a sample are staged), and the byte offset of pair (r, c) in
 // the tile is 8 (ebase + padded row TP + padded column), TP being the
-// staging area's pairs per row.
+// staging area's pairs per row.  inner = 1: the box's rows were not cut at
+// the detector's edges, so they span the rows of the tile's corners with at
+// least one row of margin on each side; every voxel of a whole z tile then
+// projects inside them (a cylindrical tile's interior exceeds its corners'
+// rows by under 0.1 row, a flat or parallel one not at all), and the circular
+// LDS kernel leaves the row clamp out for that view.
 struct alignas(16) Box {
     int clo, rlo, ncols, nrows;   // ncols == 0: the view is read from HBM
     float c0, c1, r0, r1;
-    int ebase, pad0, pad1, pad2;
+    int ebase, inner, pad1, pad2;
 };
 
 // PADZ: the tile's last z corner is taken past the volume's last slice, so
@@ -613,6 +618,8 @@ __device__ inline void tile_boxes(const BpLaunch &a, uint32_t bx, uint32_t by, u
             b.r0 = static_cast<float>(rlo + 1);
             b.r1 = below(static_cast<float>(rlo + nr));
             b.ebase = static_cast<int>(j) * (TR * TP) - (clo + 1) - (rlo + 1) * TP;
+            b.inner = static_cast<int>(floorf(rmin)) - 2 >= -1 &&
+                      static_cast<int>(floorf(rmax)) + 1 <= static_cast<int>(a.nv) + 1;
         }
     }
     boxes[j] = b;
@@ -822,60 +829,74 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                         rowb[g] = row_base<KIND, F64>(s, zb);
                         nvb[g] = nv_base<KIND, F64>(s, zb);
                     }
-                    // the v step in a vector register, so that fma(k, dnv, base) takes k as a
-                    // literal (with a scalar dnv each k != 0, 1, 2, 4 needs a move first)
-                    float dnv = s.dnv;
-                    if constexpr (KIND == GCT_CYLINDRICAL) asm("" : "=v"(dnv) : "0"(dnv));
-                    // an opaque copy of the row step: the rows are formed here, one per
-                    // update, not hoisted above the branch (16 more live registers)
-                    float drow = s.drow;
-                    asm("" : "+v"(drow));
-                    // G slices at a time: their LDS reads are issued before the first is
-                    // used, so a wave keeps several in flight (cylindrical tiles: pairs at
-                    // every z-unroll; their per-update weight takes registers of its own)
-                    constexpr int G = WIN ? 1 : (KIND == GCT_CYLINDRICAL || ZU < 32) ? 2 : kLdsGroup;
+                    // The updates of the view, with or without the row clamp (kClamp).
+                    auto updates = [&](auto clamp_t) __attribute__((always_inline)) {
+                        constexpr bool kClamp = decltype(clamp_t)::value;
+                        // the v step in a vector register, so that fma(k, dnv, base) takes k as a
+                        // literal (with a scalar dnv each k != 0, 1, 2, 4 needs a move first)
+                        float dnv = s.dnv;
+                        if constexpr (KIND == GCT_CYLINDRICAL) asm("" : "=v"(dnv) : "0"(dnv));
+                        // an opaque copy of the row step: the rows are formed here, one per
+                        // update, not hoisted above the branch (16 more live registers)
+                        float drow = s.drow;
+                        asm("" : "+v"(drow));
+                        // G slices at a time: their LDS reads are issued before the first is
+                        // used, so a wave keeps several in flight (cylindrical tiles: pairs at
+                        // every z-unroll; their per-update weight takes registers of its own)
+                        constexpr int G = WIN ? 1 : (KIND == GCT_CYLINDRICAL || ZU < 32) ? 2 : kLdsGroup;
 #pragma unroll
-                    for (int k0 = 0; k0 < ZU; k0 += G) {
-                        float2 tp[G], bt[G];
-                        float beta[G];
+                        for (int k0 = 0; k0 < ZU; k0 += G) {
+                            float2 tp[G], bt[G];
+                            float beta[G];
 #pragma unroll
-                        for (int i = 0; i < G; ++i) {
-                            const int k = k0 + i;
-                            if constexpr (WIN) {
-                                if (v < wl[k] || v >= wh[k]) continue;
+                            for (int i = 0; i < G; ++i) {
+                                const int k = k0 + i;
+                                if constexpr (WIN) {
+                                    if (v < wl[k] || v >= wh[k]) continue;
+                                }
+                                const float kk = static_cast<float>(k % static_cast<int>(kZBlock));
+                                const int g = k / static_cast<int>(kZBlock);
+                                // fma(0, drow, base) is base: slice 0 of a block takes the base as is
+                                const float rk = k % static_cast<int>(kZBlock) == 0 ? rowb[g] : fmaf(kk, drow, rowb[g]);
+                                const float row = kClamp ? __builtin_amdgcn_fmed3f(rk, rlo, rhi) : rk;
+                                beta[i] = __builtin_amdgcn_fractf(row);
+                                const uint32_t off = __umul24(static_cast<uint32_t>(row), 8u * TP) + obase;
+                                const float2 *q = reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(tile) + off);
+                                tp[i] = q[0];
+                                bt[i] = q[TP];
                             }
-                            const float kk = static_cast<float>(k % static_cast<int>(kZBlock));
-                            const int g = k / static_cast<int>(kZBlock);
-                            // fma(0, drow, base) is base: slice 0 of a block takes the base as is
-                            const float rk = k % static_cast<int>(kZBlock) == 0 ? rowb[g] : fmaf(kk, drow, rowb[g]);
-                            const float row = CLAMP ? __builtin_amdgcn_fmed3f(rk, rlo, rhi) : rk;
-                            beta[i] = __builtin_amdgcn_fractf(row);
-                            const uint32_t off = __umul24(static_cast<uint32_t>(row), 8u * TP) + obase;
-                            const float2 *q = reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(tile) + off);
-                            tp[i] = q[0];
-                            bt[i] = q[TP];
+                            // keep the group's reads ahead of its arithmetic (the scheduler
+                            // otherwise moves each read next to its use)
+                            if constexpr (G > 1) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                            for (int i = 0; i < G; ++i) {
+                                const int k = k0 + i;
+                                if constexpr (WIN) {
+                                    if (v < wl[k] || v >= wh[k]) continue;
+                                }
+                                const float kk = static_cast<float>(k % static_cast<int>(kZBlock));
+                                const int g = k / static_cast<int>(kZBlock);
+                                const float top = fmaf(alpha, tp[i].y, tp[i].x);
+                                const float bot = fmaf(alpha, bt[i].y, bt[i].x);
+                                const float val = fmaf(beta[i], bot - top, top);
+                                float w = s.w;
+                                if constexpr (KIND == GCT_CYLINDRICAL) {
+                                    const float nvk = k % static_cast<int>(kZBlock) == 0 ? nvb[g] : fmaf(kk, dnv, nvb[g]);
+                                    w = __builtin_amdgcn_rcpf(fmaf(nvk, nvk, s.rho2));
+                                }
+                                acc[k] = fmaf(val, w, acc[k]);
+                            }
                         }
-                        // keep the group's reads ahead of its arithmetic (the scheduler
-                        // otherwise moves each read next to its use)
-                        if constexpr (G > 1) __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                        for (int i = 0; i < G; ++i) {
-                            const int k = k0 + i;
-                            if constexpr (WIN) {
-                                if (v < wl[k] || v >= wh[k]) continue;
-                            }
-                            const float kk = static_cast<float>(k % static_cast<int>(kZBlock));
-                            const int g = k / static_cast<int>(kZBlock);
-                            const float top = fmaf(alpha, tp[i].y, tp[i].x);
-                            const float bot = fmaf(alpha, bt[i].y, bt[i].x);
-                            const float val = fmaf(beta[i], bot - top, top);
-                            float w = s.w;
-                            if constexpr (KIND == GCT_CYLINDRICAL) {
-                                const float nvk = k % static_cast<int>(kZBlock) == 0 ? nvb[g] : fmaf(kk, dnv, nvb[g]);
-                                w = __builtin_amdgcn_rcpf(fmaf(nvk, nvk, s.rho2));
-                            }
-                            acc[k] = fmaf(val, w, acc[k]);
-                        }
+                    };
+                    // A whole z tile whose box the detector's edges did not cut needs no
+                    // row clamp (Box::inner); the flag is the same for the whole workgroup.
+                    if constexpr (CLAMP && !WIN) {
+                        if (__builtin_amdgcn_readfirstlane(b.inner) != 0 && iz0 + static_cast<uint32_t>(ZU) <= a.nz)
+                            updates(std::false_type{});
+                        else
+                            updates(std::true_type{});
+                    } else {
+                        updates(std::bool_constant<CLAMP>{});
                     }
                 } else {
                     const T *p = base + static_cast<size_t>(v) * view_elems;
