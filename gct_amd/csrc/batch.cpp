@@ -133,8 +133,11 @@ struct gct_batch {
     hipEvent_t staged = nullptr;      // recorded after the last upload from h_stage
     hipEvent_t plain_done = nullptr;  // recorded after the last plain execute's work
     hipEvent_t graph_done = nullptr;  // recorded after the last graph launch
-    // Groups run as concurrent branches: forked from the launching stream to
-    // side streams by events and joined back the same way (also under capture).
+    // Groups too small to fill the device run as concurrent branches: forked
+    // from the launching stream to side streams by events and joined back the
+    // same way (also under capture); grid_cap is the persistent grid's size,
+    // 8 workgroups per CU (launch_backproject_batch_group).
+    uint32_t grid_cap = 2048;
     hipStream_t side[kSideStreams] = {};
     hipEvent_t fork = nullptr, join[kSideStreams] = {};
     hipGraph_t graph = nullptr;
@@ -169,15 +172,26 @@ void batch_record(gct_batch &b, const float *const *proj, const BatchScan *scans
     const uint32_t n = uint32_t(b.plans.size());
     batch_filters(b.plans.data(), b.order.data(), n, proj, s);
     launch_zero_counters(counters, uint32_t(b.groups.size()), s);
-    if (b.groups.size() == 1) {
-        launch_backproject_batch_group(scans, b.groups[0], counters, s);
-        return;
-    }
-    hip_check(hipEventRecord(b.fork, s), "fork");
-    const uint32_t used = std::min<uint32_t>(gct_batch::kSideStreams, uint32_t(b.groups.size()));
-    for (uint32_t i = 0; i < used; ++i) hip_check(hipStreamWaitEvent(b.side[i], b.fork, 0), "fork wait");
+    // A group with at least as many tiles as its persistent grid has
+    // workgroups fills the device by itself: those run one after another on
+    // the launching stream.  Run beside each other, two such kernels only
+    // share the CUs' workgroup slots, and the mix ran slower than the sum
+    // (uneven batch: 0.212 s with the 2-D group on a side stream, 0.191 s in
+    // sequence).  Groups too small to fill the device are forked to side
+    // streams first, so they overlap the large ones and each other.
+    std::vector<size_t> small;
     for (size_t gi = 0; gi < b.groups.size(); ++gi)
-        launch_backproject_batch_group(scans, b.groups[gi], counters + gi, b.side[gi % gct_batch::kSideStreams]);
+        if (b.groups[gi].total_tiles < b.grid_cap) small.push_back(gi);
+    const uint32_t used = std::min<uint32_t>(gct_batch::kSideStreams, uint32_t(small.size()));
+    if (used > 0) {
+        hip_check(hipEventRecord(b.fork, s), "fork");
+        for (uint32_t i = 0; i < used; ++i) hip_check(hipStreamWaitEvent(b.side[i], b.fork, 0), "fork wait");
+        for (size_t i = 0; i < small.size(); ++i)
+            launch_backproject_batch_group(scans, b.groups[small[i]], counters + small[i],
+                                           b.side[i % gct_batch::kSideStreams]);
+    }
+    for (size_t gi = 0; gi < b.groups.size(); ++gi)
+        if (b.groups[gi].total_tiles >= b.grid_cap) launch_backproject_batch_group(scans, b.groups[gi], counters + gi, s);
     for (uint32_t i = 0; i < used; ++i) {
         hip_check(hipEventRecord(b.join[i], b.side[i]), "join");
         hip_check(hipStreamWaitEvent(s, b.join[i], 0), "join wait");
@@ -201,6 +215,9 @@ extern "C" gct_status gct_batch_create(gct_plan *const *plans, uint32_t n, gct_b
         b->groups.resize(batch_groups(plans, b->order.data(), n, b->groups.data(), n));
         b->table.resize(n);
         DeviceGuard dg(b->device);
+        int cus = 256;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess) cus = 256;
+        b->grid_cap = uint32_t(cus) * 8u;
         const size_t tbytes = size_t(n) * sizeof(BatchScan), cbytes = b->groups.size() * sizeof(uint32_t);
         b->d_plain = DevBuf(tbytes);
         b->d_graph = DevBuf(tbytes);
