@@ -65,6 +65,12 @@ F_ALG = {"parallel_1d": 13, "fan_line_1d": 22, "fan_arc_1d": 26, "parallel_2d": 
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 N_CU, CLOCK_HZ = 256, 2.4e9   # MI355X: one wave64 VALU instruction per CU per cycle (4 SIMD16, non-packed)
+LDS_BYTES_PER_CU_CYCLE = 128  # measured LDS read rate (ds_read2_b64: 1 KiB per wave instruction in 8 cycles)
+LDS_READ_BYTES_PER_UPDATE = 16   # the LDS backprojection reads two (value, difference) pairs per update
+
+
+def variant_is_lds(g, info):
+    return info.bp_kernel == g.BP_LDS
 
 
 def host_cores():
@@ -251,7 +257,7 @@ def run_flat(a, g, W, torch, np, sections):
     # the kernel the tuned plan launches, named as rocprofv3 names it; the
     # counter-measured traffic applies only when the profile is of that kernel
     kernel = _bp_kernel_name(g, info)
-    traffic, traffic_kernel, valu_insts = None, None, None
+    traffic, traffic_kernel, valu_insts, lds_busy, dual = None, None, None, None, None
     tfile = ROOT / "profiles" / "bp_flat_cone_256_traffic.json"
     if tfile.exists():
         try:
@@ -260,8 +266,13 @@ def run_flat(a, g, W, torch, np, sections):
             if traffic_kernel == kernel:
                 traffic = tj.get("hbm_bytes_per_launch")
                 valu_insts = tj.get("valu_insts_per_launch")
+                lds_busy = tj.get("lds_busy_frac")
+                dual = tj.get("valu_dual_issued_frac")
         except (OSError, ValueError):
             traffic = None
+    # LDS read rate of the launch against 128 B per CU-cycle at the nominal clock
+    lds_read_frac = round(LDS_READ_BYTES_PER_UPDATE * upd / (bp_mean * 1e-3) /
+                          (LDS_BYTES_PER_CU_CYCLE * N_CU * CLOCK_HZ), 4) if variant_is_lds(g, info) else None
     # VALU issue utilisation: the profiled kernel's wave instructions over the
     # issue slots of this run's launch (one per CU per cycle at 2.4 GHz)
     valu_issue = round(valu_insts / (N_CU * CLOCK_HZ * bp_mean * 1e-3), 4) if valu_insts else None
@@ -340,10 +351,16 @@ def run_flat(a, g, W, torch, np, sections):
             "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 4),
             "frac_basis": "algorithmic FLOPs of the reference's per-update formula (45 per update) over the "
                           "FP32 peak; the kernel evaluates them strength-reduced along z in fewer instructions, "
-                          "so its own limit is valu_issue_frac",
+                          "so its own limits are valu_issue_frac and lds_read_frac",
             "valu_issue_frac": valu_issue,
             # wave instructions x 64 lanes over the updates (one lane per update)
             "valu_insts_per_update": round(valu_insts * 64 / upd, 2) if valu_insts else None,
+            "valu_dual_issued_frac_profiled": dual,
+            "lds_read_frac": lds_read_frac,
+            "lds_busy_frac_profiled": lds_busy,
+            "binding": "LDS bandwidth: 16 B of LDS reads per update at 128 B per CU-cycle (lds_read_frac; the "
+                       "profiled LDS busy fraction includes staging writes and bank conflicts); VALU issue has "
+                       "headroom because full-rate FP32 instructions dual-issue",
             "traffic": traffic,
             "traffic_profiled_kernel": traffic_kernel,
             "alg_flops_per_update": F_ALG_FLAT,
