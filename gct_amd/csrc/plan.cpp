@@ -305,7 +305,9 @@ bool rows_inside_detector(const gct_plan &p, const std::vector<BpRec64> &rec64)
 // Whether the arc kinds' FP32 kernels may take the fan angle without octant
 // reduction (fan_atan_narrow): in every view every voxel lies in front of the
 // source (nw > 0: nw is affine over the image, so its minimum is at a corner of
-// the image's footprint, the z term included for a cylindrical detector), and
+// the image's footprint, the z term included for a cylindrical detector; with a
+// margin of 1e-5 of the form's magnitude, so that the FP32 form the kernels
+// evaluate is positive too), and
 // both padded edge columns of the detector, 0 and nu + 1, lie within the
 // clamped angle's reach with half a column to spare -- then a voxel whose angle
 // is clamped projects outside the detector, as its exact angle does.  In FP64
@@ -319,9 +321,10 @@ bool narrow_fan_ok(const gct_plan &p, const std::vector<BpRec64> &rec64)
     const double xm = double(p.img.x.num_samples) - 1.0, ym = double(p.img.y.num_samples) - 1.0;
     const double zm = (p.img.dim == 3) ? double(p.img.z.num_samples) - 1.0 : 0.0;
     for (const BpRec64 &r : rec64) {
+        const double mag = std::fabs(r.w[0]) + std::fabs(r.w[1]) * xm + std::fabs(r.w[2]) * ym + std::fabs(r.w[3]) * zm;
         for (int c = 0; c < 8; ++c) {
             const double x = (c & 1) ? xm : 0.0, y = (c & 2) ? ym : 0.0, z = (c & 4) ? zm : 0.0;
-            if (!(r.w[0] + r.w[1] * x + r.w[2] * y + r.w[3] * z > 0.0)) return false;
+            if (!(r.w[0] + r.w[1] * x + r.w[2] * y + r.w[3] * z > 1e-5 * mag)) return false;
         }
         // padded column = gamma / dgamma + k0 (k0 = centre + 1)
         const double dg = g.det_row.sampling_width;
