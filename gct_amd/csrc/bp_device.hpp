@@ -43,6 +43,49 @@ template <> struct Stor<GCT_STORE_BF16> {
     }
 };
 
+// The horizontal difference t1 - t0 of two neighbouring filtered values, as
+// every backprojection path takes it: in FP32, and for FP16 storage rounded to
+// FP16 once (what the LDS kernels stage, LdsPair below), so that all paths of
+// a storage form give the same bits.
+template <int S>
+__device__ inline float tap_diff(float t0, float t1)
+{
+    if constexpr (S == GCT_STORE_F16)
+        return static_cast<float>(static_cast<_Float16>(t1 - t0));
+    else
+        return t1 - t0;
+}
+
+// A staged detector pixel of the LDS kernels: its value and the difference to
+// its right neighbour.  Two FP32 values (8 B); for FP16 storage both in FP16
+// (4 B: the value as stored, the difference rounded once), so that a bilinear
+// update reads 8 B of LDS instead of 16 -- the LDS bandwidth binds these
+// kernels -- and a horizontal tap is one v_fma_mix_f32 (FP16 operands, FP32
+// arithmetic: the same bits as the FP32 FMA of the widened values).
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+template <int S> struct LdsPair {
+    using T = float2;
+    static constexpr uint32_t kBytes = 8;
+    __device__ static T make(float v, float d) { return make_float2(v, d); }
+    __device__ static float lerp(float alpha, T p) { return fmaf(alpha, p.y, p.x); }
+};
+template <> struct LdsPair<GCT_STORE_F16> {
+    using T = uint32_t;
+    static constexpr uint32_t kBytes = 4;
+    __device__ static T make(float v, float d)
+    {
+        half2_t h;
+        h.x = static_cast<_Float16>(v);   // exact: v is an FP16 value widened
+        h.y = static_cast<_Float16>(d);   // tap_diff's rounding
+        return __builtin_bit_cast(uint32_t, h);
+    }
+    __device__ static float lerp(float alpha, T p)
+    {
+        const half2_t h = __builtin_bit_cast(half2_t, p);
+        return fmaf(alpha, static_cast<float>(h.y), static_cast<float>(h.x));
+    }
+};
+
 // ---- backprojection -------------------------------------------------------
 
 // Fan angle atan2(y, x) for the FP32 arc-detector forms.  The octant is
@@ -142,8 +185,8 @@ __device__ inline float bilinear(const typename Stor<S>::T *__restrict__ p, uint
     const auto *q = tap<S>(p, static_cast<uint32_t>(r) * pitch + cs.colbase);
     const float t0 = Stor<S>::ld(q, 0), t1 = Stor<S>::ld(q, 1);
     const float b0 = Stor<S>::ld(q + pitch, 0), b1 = Stor<S>::ld(q + pitch, 1);
-    const float top = fmaf(cs.alpha, t1 - t0, t0);
-    const float bot = fmaf(cs.alpha, b1 - b0, b0);
+    const float top = fmaf(cs.alpha, tap_diff<S>(t0, t1), t0);
+    const float bot = fmaf(cs.alpha, tap_diff<S>(b0, b1), b0);
     return fmaf(beta, bot - top, top);
 }
 
@@ -514,10 +557,11 @@ __global__ __launch_bounds__(64) void bp2d_kernel(BpLaunch a)
 // projects the tile's 8 corner voxels, takes the bounding box of the detector
 // pixels they reach (plus one pixel of margin), and stages that box from HBM
 // into LDS as pairs (p[r][c], p[r][c+1] - p[r][c]) -- value and horizontal
-// difference, FP16/BF16 widened to FP32 -- so each thread's bilinear tap is
-// two 64-bit LDS reads and one FMA per row.  The differences are the same
-// FP32 subtractions the direct kernel performs per update, so both kernels
-// give the same bits.  Coordinates are clamped to the staged box: the box never
+// difference, FP32 for FP32 and BF16 storage (BF16 widened), FP16 for FP16
+// storage (LdsPair) -- so each thread's bilinear tap is two LDS reads of a
+// pair (one ds_read2) and one FMA per row.  The differences are those the
+// direct kernel forms per update (tap_diff), so both kernels give the same
+// bits.  Coordinates are clamped to the staged box: the box never
 // extends past the zero padding of the filtered-projection buffer, so the
 // clamp also reproduces the border-zero texture, and it keeps every LDS read
 // inside the box.  A view whose box does not fit is read from HBM directly.
@@ -668,21 +712,21 @@ __device__ inline void stage_load(__amdgpu_buffer_rsrc_t rs, uint32_t row0, uint
     }
 }
 
-template <int TP, int RSTEP, int NIT, bool HALF>
-__device__ inline void stage_store(float2 *tl, int rw, int nrows, uint32_t lane, const float *v)
+template <int S, int TP, int RSTEP, int NIT, bool HALF>
+__device__ inline void stage_store(typename LdsPair<S>::T *tl, int rw, int nrows, uint32_t lane, const float *v)
 {
     const uint32_t c = HALF ? (lane & 31u) : lane;
     const uint32_t h = HALF ? (lane >> 5) : 0u;
     const bool ok_all = c < static_cast<uint32_t>(TP);   // both rows of a HALF step are in the box
     const bool ok_top = ok_all && h == 0u;              // only the upper one is
-    float2 *tw = tl + (static_cast<uint32_t>(rw) + h) * TP + c;
+    typename LdsPair<S>::T *tw = tl + (static_cast<uint32_t>(rw) + h) * TP + c;
 #pragma unroll
     for (int it = 0; it < NIT; ++it) {
         const int r = rw + it * RSTEP;
         if (r < nrows) {
             const float d = next_lane(v[it]) - v[it];
             const bool both = !HALF || r + 1 < nrows;
-            if (both ? ok_all : ok_top) tw[it * RSTEP * TP] = make_float2(v[it], d);
+            if (both ? ok_all : ok_top) tw[it * RSTEP * TP] = LdsPair<S>::make(v[it], d);
         }
     }
 }
@@ -693,7 +737,7 @@ __device__ inline void stage_store(float2 *tl, int rw, int nrows, uint32_t lane,
 // 32 pairs per row is staged two rows per wave.
 template <int S, int TR, int VB, int TP>
 __device__ inline void stage_block(const typename Stor<S>::T *base, size_t view_elems, uint32_t pitch, uint32_t vb0,
-                                   uint32_t nvb, const Box *bx, float2 *tile, int wave, uint32_t lane)
+                                   uint32_t nvb, const Box *bx, typename LdsPair<S>::T *tile, int wave, uint32_t lane)
 {
     using T = typename Stor<S>::T;
     // a view and the padded row after it (the next view's first, or the buffer's spare one)
@@ -710,19 +754,19 @@ __device__ inline void stage_block(const typename Stor<S>::T *base, size_t view_
             0x00020000);
         const uint32_t row0 = (static_cast<uint32_t>(rlo + 1) * pitch + static_cast<uint32_t>(clo + 1)) *
                               static_cast<uint32_t>(sizeof(T));
-        float2 *tl = tile + j * (TR * TP);
+        typename LdsPair<S>::T *tl = tile + j * (TR * TP);
         if (ncols - 1 < 32) {
             constexpr int kN = (TR + 7) / 8;
             float v[kN];
             const uint32_t voff = ((lane >> 5) * pitch + (lane & 31u)) * static_cast<uint32_t>(sizeof(T));
             stage_load<S, 8, kN, true>(rs, row0, row_bytes, voff, 2 * wave, nrows, v);
-            stage_store<TP, 8, kN, true>(tl, 2 * wave, nrows, lane, v);
+            stage_store<S, TP, 8, kN, true>(tl, 2 * wave, nrows, lane, v);
         } else {
             constexpr int kW = (TR + 3) / 4;
             float v[kW];
             const uint32_t voff = lane * static_cast<uint32_t>(sizeof(T));
             stage_load<S, 4, kW, false>(rs, row0, row_bytes, voff, wave, nrows, v);
-            stage_store<TP, 4, kW, false>(tl, wave, nrows, lane, v);
+            stage_store<S, TP, 4, kW, false>(tl, wave, nrows, lane, v);
         }
     }
 }
@@ -742,6 +786,9 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                               Box *boxes)
 {
     using T = typename Stor<S>::T;
+    using Pair = typename LdsPair<S>::T;   // the staging area holds pairs of this type (kPB bytes)
+    constexpr uint32_t kPB = LdsPair<S>::kBytes;
+    Pair *ptile = reinterpret_cast<Pair *>(tile);
     // Each wave takes an 8 x 8 quadrant of the tile, and each 16 lanes of it a
     // 4 x 4 block: the lanes' detector columns then span about a quarter as
     // many LDS banks per view as a 16-voxel strip's do.
@@ -805,7 +852,7 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
         constexpr bool kGeneral = decltype(general)::value;
         const uint32_t nvb = min(static_cast<uint32_t>(VB), vend - vb0);
         const Box *bc = boxes + cur * VB;
-        stage_block<S, TR, VB, TP>(base, view_elems, a.pitch, vb0, nvb, bc, tile, wave, lane);
+        stage_block<S, TR, VB, TP>(base, view_elems, a.pitch, vb0, nvb, bc, ptile, wave, lane);
         __syncthreads();
         if (active) {
             for (uint32_t j = 0; j < nvb; ++j) {
@@ -820,7 +867,7 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                     // The per-view terms of the byte offset fold into one base (modulo 2^32:
                     // it is negative when the box starts below row 0 of view j's slot); the
                     // padded row index is added with one 24-bit multiply-add.
-                    const uint32_t obase = 8u * static_cast<uint32_t>(b.ebase + static_cast<int>(clp));
+                    const uint32_t obase = kPB * static_cast<uint32_t>(b.ebase + static_cast<int>(clp));
                     const float rlo = b.r0, rhi = b.r1;
                     float rowb[kZB], nvb[kZB];
 #pragma unroll
@@ -846,7 +893,7 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                         constexpr int G = WIN ? 1 : (KIND == GCT_CYLINDRICAL || ZU < 32) ? 2 : kLdsGroup;
 #pragma unroll
                         for (int k0 = 0; k0 < ZU; k0 += G) {
-                            float2 tp[G], bt[G];
+                            Pair tp[G], bt[G];
                             float beta[G];
 #pragma unroll
                             for (int i = 0; i < G; ++i) {
@@ -860,8 +907,8 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                                 const float rk = k % static_cast<int>(kZBlock) == 0 ? rowb[g] : fmaf(kk, drow, rowb[g]);
                                 const float row = kClamp ? __builtin_amdgcn_fmed3f(rk, rlo, rhi) : rk;
                                 beta[i] = __builtin_amdgcn_fractf(row);
-                                const uint32_t off = __umul24(static_cast<uint32_t>(row), 8u * TP) + obase;
-                                const float2 *q = reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(tile) + off);
+                                const uint32_t off = __umul24(static_cast<uint32_t>(row), kPB * TP) + obase;
+                                const Pair *q = reinterpret_cast<const Pair *>(reinterpret_cast<const char *>(ptile) + off);
                                 tp[i] = q[0];
                                 bt[i] = q[TP];
                             }
@@ -876,8 +923,8 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                                 }
                                 const float kk = static_cast<float>(k % static_cast<int>(kZBlock));
                                 const int g = k / static_cast<int>(kZBlock);
-                                const float top = fmaf(alpha, tp[i].y, tp[i].x);
-                                const float bot = fmaf(alpha, bt[i].y, bt[i].x);
+                                const float top = LdsPair<S>::lerp(alpha, tp[i]);
+                                const float bot = LdsPair<S>::lerp(alpha, bt[i]);
                                 const float val = fmaf(beta[i], bot - top, top);
                                 float w = s.w;
                                 if constexpr (KIND == GCT_CYLINDRICAL) {
@@ -999,6 +1046,9 @@ __device__ void bp_tile3d_lds_nc(const BpLaunch &a, uint32_t bx, uint32_t by, ui
 {
     constexpr int ZU = static_cast<int>(kZBlock);
     using T = typename Stor<S>::T;
+    using Pair = typename LdsPair<S>::T;   // the staging area holds pairs of this type (kPB bytes)
+    constexpr uint32_t kPB = LdsPair<S>::kBytes;
+    Pair *ptile = reinterpret_cast<Pair *>(tile);
     const uint32_t ix = bx * 16u + ((tid >> 6) & 1u) * 8u + ((tid >> 4) & 1u) * 4u + (tid & 3u);
     const uint32_t iy = by * 16u + (tid >> 7) * 8u + ((tid >> 5) & 1u) * 4u + ((tid >> 2) & 3u);
     const uint32_t iz0 = a.z0 + bz * ZU;   // global slice index, the first of a z block
@@ -1030,7 +1080,7 @@ __device__ void bp_tile3d_lds_nc(const BpLaunch &a, uint32_t bx, uint32_t by, ui
         constexpr bool kGeneral = decltype(general)::value;
         const uint32_t nvb = min(static_cast<uint32_t>(VB), vend - vb0);
         const Box *bc = boxes + cur * VB;
-        stage_block<S, TR, VB, TP>(base, view_elems, a.pitch, vb0, nvb, bc, tile, wave, lane);
+        stage_block<S, TR, VB, TP>(base, view_elems, a.pitch, vb0, nvb, bc, ptile, wave, lane);
         __syncthreads();
         if (active) {
             for (uint32_t j = 0; j < nvb; ++j) {
@@ -1038,12 +1088,12 @@ __device__ void bp_tile3d_lds_nc(const BpLaunch &a, uint32_t bx, uint32_t by, ui
                 const NcBase nb = nc_base<F64>(a, v, fx, fy, fz);
                 const Box b = bc[j];
                 if (!kGeneral || b.ncols != 0) {
-                    // byte offset of pair (r, c): 8 (ebase + c) + 8 TP r, modulo 2^32
-                    const uint32_t eb = 8u * static_cast<uint32_t>(b.ebase);
+                    // byte offset of pair (r, c): kPB (ebase + c) + kPB TP r, modulo 2^32
+                    const uint32_t eb = kPB * static_cast<uint32_t>(b.ebase);
                     constexpr int G = 2;
 #pragma unroll
                     for (int k0 = 0; k0 < ZU; k0 += G) {
-                        float2 tp[G], bt[G];
+                        Pair tp[G], bt[G];
                         float alpha[G], beta[G], w[G];
 #pragma unroll
                         for (int i = 0; i < G; ++i) {
@@ -1054,17 +1104,17 @@ __device__ void bp_tile3d_lds_nc(const BpLaunch &a, uint32_t bx, uint32_t by, ui
                             const float rc = __builtin_amdgcn_fmed3f(row, b.r0, b.r1);
                             alpha[i] = __builtin_amdgcn_fractf(cc);
                             beta[i] = __builtin_amdgcn_fractf(rc);
-                            const uint32_t off = __umul24(static_cast<uint32_t>(rc), 8u * TP) + eb +
-                                                 8u * static_cast<uint32_t>(cc);
-                            const float2 *q = reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(tile) + off);
+                            const uint32_t off = __umul24(static_cast<uint32_t>(rc), kPB * TP) + eb +
+                                                 kPB * static_cast<uint32_t>(cc);
+                            const Pair *q = reinterpret_cast<const Pair *>(reinterpret_cast<const char *>(ptile) + off);
                             tp[i] = q[0];
                             bt[i] = q[TP];
                         }
                         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                         for (int i = 0; i < G; ++i) {
-                            const float top = fmaf(alpha[i], tp[i].y, tp[i].x);
-                            const float bot = fmaf(alpha[i], bt[i].y, bt[i].x);
+                            const float top = LdsPair<S>::lerp(alpha[i], tp[i]);
+                            const float bot = LdsPair<S>::lerp(alpha[i], bt[i]);
                             const float val = fmaf(beta[i], bot - top, top);
                             acc[k0 + i] = fmaf(val, w[i], acc[k0 + i]);
                         }

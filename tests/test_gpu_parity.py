@@ -260,6 +260,10 @@ def test_projection_matrix_geometry_tilted_detector(torch_dev, setup, nv):
     direct, dplan = run_plan(torch_dev, wl, proj, setup_precision=setup, bp_kernel=g.BP_DIRECT)
     assert dplan.info().bp_kernel == g.BP_DIRECT
     assert np.array_equal(direct, got)
+    # FP16 storage: the kernel of this case stages FP16 pairs as the circular one does
+    half, _ = run_plan(torch_dev, wl, proj, setup_precision=setup, storage=g.STORE_F16)
+    half_direct, _ = run_plan(torch_dev, wl, proj, setup_precision=setup, storage=g.STORE_F16, bp_kernel=g.BP_DIRECT)
+    assert np.array_equal(half, half_direct)
     low = g.reconstruct(wl.scan, proj, wl.img, mem_mode=g.MEM_LOW, view_chunk=17, setup_precision=setup)
     assert np.array_equal(low, got)
     img = g.Image(48, 48, 0.5, 0.5, 80, 0.3, mid=(0.3, -0.2, 0.1))
@@ -345,10 +349,11 @@ def test_z_slab_options(torch_dev):
     assert np.array_equal(g.reconstruct(w2.scan, p2, w2.img, mem_mode=g.MEM_SLAB), high)
 
 
+@pytest.mark.parametrize("storage", [g.STORE_F32, g.STORE_F16, g.STORE_BF16], ids=["f32", "f16", "bf16"])
 @pytest.mark.parametrize("nv", [80, 40], ids=["rows_inside", "rows_cut"])
 @pytest.mark.parametrize("setup", [g.SETUP_F32, g.SETUP_F64], ids=["f32", "f64"])
 @pytest.mark.parametrize("kind", KINDS, ids=g.KIND_NAMES)
-def test_lds_and_direct_kernels_identical(torch_dev, kind, setup, nv):
+def test_lds_and_direct_kernels_identical(torch_dev, kind, setup, nv, storage):
     """The LDS-staged and the direct backprojector read the same taps and run
     the same arithmetic per update, and every voxel's row coordinate is
     evaluated from its 16-slice z block, so they agree bit for bit whatever the
@@ -356,10 +361,14 @@ def test_lds_and_direct_kernels_identical(torch_dev, kind, setup, nv):
     detector rows the 64^3 volume projects inside the detector in every view
     (the plan reports rows_inside and the LDS kernel runs without its row
     clamp); with 40 rows it does not, the staged boxes are cut by the
-    detector's edges and the clamp reproduces the zero border."""
+    detector's edges and the clamp reproduces the zero border.  For every
+    storage form: FP16 storage stages FP16 (value, difference) pairs, the
+    difference rounded once to FP16 as the direct kernel rounds it."""
     torch = torch_dev
     if g.is_1d(kind) and nv != 80:
         pytest.skip("2-D images have no detector rows")
+    if storage != g.STORE_F32 and (g.is_1d(kind) or setup == g.SETUP_F64):
+        pytest.skip("storage forms are compared on the 3-D kinds with FP32 set-up")
     wl = W.small_case(kind, nv=nv)
     proj = torch.from_numpy(wl.projections()).cuda()
     out = []
@@ -368,7 +377,7 @@ def test_lds_and_direct_kernels_identical(torch_dev, kind, setup, nv):
             continue
         vb, rows, pairs = shape if shape and not g.is_1d(kind) else (0, 0, 0)
         plan = g.Plan(wl.scan, wl.img, bp_kernel=bk, z_unroll=0 if g.is_1d(kind) else zu, view_block=vb,
-                      lds_rows=rows, lds_pairs=pairs, setup_precision=setup)
+                      lds_rows=rows, lds_pairs=pairs, setup_precision=setup, storage=storage)
         if kind in (g.FLAT, g.CYLINDRICAL):
             assert plan.info().setup_f64 == (setup == g.SETUP_F64)
         if bk == g.BP_LDS and not g.is_1d(kind):
