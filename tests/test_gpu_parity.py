@@ -349,11 +349,24 @@ def test_z_slab_options(torch_dev):
     assert np.array_equal(g.reconstruct(w2.scan, p2, w2.img, mem_mode=g.MEM_SLAB), high)
 
 
-@pytest.mark.parametrize("storage", [g.STORE_F32, g.STORE_F16, g.STORE_BF16], ids=["f32", "f16", "bf16"])
-@pytest.mark.parametrize("nv", [80, 40], ids=["rows_inside", "rows_cut"])
-@pytest.mark.parametrize("setup", [g.SETUP_F32, g.SETUP_F64], ids=["f32", "f64"])
-@pytest.mark.parametrize("kind", KINDS, ids=g.KIND_NAMES)
-def test_lds_and_direct_kernels_identical(torch_dev, kind, setup, nv, storage):
+# (kind, set-up, detector rows, storage): every kind with FP32 storage (2-D
+# images have no detector rows, so one row count), and the 3-D kinds with FP16
+# and BF16 storage at FP32 set-up.
+_LDS_CASES = ([(k, st, nv, g.STORE_F32) for k in KINDS for st in (g.SETUP_F32, g.SETUP_F64)
+               for nv in ((80,) if g.is_1d(k) else (80, 40))] +
+              [(k, g.SETUP_F32, nv, sto) for k in KINDS if not g.is_1d(k) for nv in (80, 40)
+               for sto in (g.STORE_F16, g.STORE_BF16)])
+
+
+def _lds_case_id(c):
+    k, st, nv, sto = c
+    return "-".join([g.KIND_NAMES[k], {g.SETUP_F32: "f32", g.SETUP_F64: "f64"}[st],
+                     "rows_inside" if nv == 80 else "rows_cut", {g.STORE_F32: "f32", g.STORE_F16: "f16",
+                                                                  g.STORE_BF16: "bf16"}[sto]])
+
+
+@pytest.mark.parametrize("case", _LDS_CASES, ids=_lds_case_id)
+def test_lds_and_direct_kernels_identical(torch_dev, case):
     """The LDS-staged and the direct backprojector read the same taps and run
     the same arithmetic per update, and every voxel's row coordinate is
     evaluated from its 16-slice z block, so they agree bit for bit whatever the
@@ -365,10 +378,7 @@ def test_lds_and_direct_kernels_identical(torch_dev, kind, setup, nv, storage):
     storage form: FP16 storage stages FP16 (value, difference) pairs, the
     difference rounded once to FP16 as the direct kernel rounds it."""
     torch = torch_dev
-    if g.is_1d(kind) and nv != 80:
-        pytest.skip("2-D images have no detector rows")
-    if storage != g.STORE_F32 and (g.is_1d(kind) or setup == g.SETUP_F64):
-        pytest.skip("storage forms are compared on the 3-D kinds with FP32 set-up")
+    kind, setup, nv, storage = case
     wl = W.small_case(kind, nv=nv)
     proj = torch.from_numpy(wl.projections()).cuda()
     out = []
