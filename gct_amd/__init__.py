@@ -91,7 +91,7 @@ class PlanInfo(C.Structure):
                 ("rot_dir", C.c_int32), ("setup_f64", C.c_int32), ("circular", C.c_int32), ("storage", C.c_int32),
                 ("bp_kernel", C.c_int32), ("view_block", C.c_uint32), ("lds_rows", C.c_uint32),
                 ("lds_pairs", C.c_uint32), ("rows_inside", C.c_int32), ("round_robin", C.c_int32),
-                ("narrow_fan", C.c_int32)]
+                ("narrow_fan", C.c_int32), ("box_inner", C.c_int32)]
 
 
 class ServerOpts(C.Structure):

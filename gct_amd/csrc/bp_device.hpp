@@ -603,9 +603,10 @@ __device__ inline float max8(float x)
 // staging area's pairs per row.  inner = 1: the box's rows were not cut at
 // the detector's edges, so they span the rows of the tile's corners with at
 // least one row of margin on each side; every voxel of a whole z tile then
-// projects inside them (a cylindrical tile's interior exceeds its corners'
-// rows by under 0.1 row, a flat or parallel one not at all), and the circular
-// LDS kernel leaves the row clamp out for that view.
+// projects inside them when the plan has bounded how far a tile's interior
+// exceeds its corners' rows (BpLaunch::box_inner: not at all for flat and
+// parallel rows, under half a row for cylindrical ones), and the circular LDS
+// kernel leaves the row clamp out for that view.
 struct alignas(16) Box {
     int clo, rlo, ncols, nrows;   // ncols == 0: the view is read from HBM
     float c0, c1, r0, r1;
@@ -938,7 +939,8 @@ __device__ void bp_tile3d_lds(const BpLaunch &a, uint32_t bx, uint32_t by, uint3
                     // A whole z tile whose box the detector's edges did not cut needs no
                     // row clamp (Box::inner); the flag is the same for the whole workgroup.
                     if constexpr (CLAMP && !WIN) {
-                        if (__builtin_amdgcn_readfirstlane(b.inner) != 0 && iz0 + static_cast<uint32_t>(ZU) <= a.nz)
+                        if (a.box_inner && __builtin_amdgcn_readfirstlane(b.inner) != 0 &&
+                            iz0 + static_cast<uint32_t>(ZU) <= a.nz)
                             updates(std::false_type{});
                         else
                             updates(std::true_type{});

@@ -158,6 +158,7 @@ struct BpLaunch {
     bool rows_inside;             // every voxel's row lies inside the detector: LDS kernel without row clamp
     bool narrow_fan;              // arc kinds, FP32 set-up: every voxel in front of the source and the fan
                                   //   within +-0.6 rad (fan_atan_narrow instead of the octant-reduced form)
+    bool box_inner;               // a tile's voxels stay within a row of its corners' rows (Box::inner applies)
     bool round_robin;             // LDS kernel: tiles dealt to the XCDs round robin, not in contiguous runs
     // The launch covers slices [z0, z0 + zcount) of the nz-slice volume and `vol`
     // holds just those (a z slab; z0 a multiple of 32, so tiles and their boxes

@@ -851,11 +851,14 @@ void joseph_lds_kernel(FpLaunch a, uint32_t tiles_c, uint32_t tiles_r)
             // wave q stages (plane, row) pairs q, q + 4, ... -- pair 4 k + q is plane k / 4, row
             // 4 (k % 4) + q -- one tap per lane, whole rows of 64 lanes and all kJB planes whatever
             // the box: elements past its columns, rows or planes land in the brick and are never
-            // read (the traced taps stay within columns < W, rows < H and planes < np).  The row's
-            // element index is wave-uniform (scalar); the loads are raw buffer loads with the whole
-            // byte offset in the lane's (range-checked) operand, so an element past the copies
-            // reads 0 instead of faulting.  In two halves: 16 loads in flight per lane, then their
-            // stores, each at an immediate offset from the lane's column.
+            // read (the traced taps stay within columns < W, rows < H and planes < np).  Skipping
+            // the rows past the box (by scalar branches, or by an offset the range check drops)
+            // fetched a third of the bytes but took longer: the per-row selects cost more issue
+            // than the extra loads cost bandwidth.  The row's element index is wave-uniform
+            // (scalar); the loads are raw buffer loads with the whole byte offset in the lane's
+            // (range-checked) operand, so an element past the copies reads 0 instead of faulting.
+            // In two halves: 16 loads in flight per lane, then their stores, each at an immediate
+            // offset from the lane's column.
             const uint32_t wq = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(wave)));
             const uint32_t alo_s = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(alo)));
             const uint32_t blo_s = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(blo)));

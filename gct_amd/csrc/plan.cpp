@@ -302,6 +302,68 @@ bool rows_inside_detector(const gct_plan &p, const std::vector<BpRec64> &rec64)
     return true;
 }
 
+// Whether the LDS kernel may leave the row clamp out for a view whose staged
+// box the detector's edges did not cut (Box::inner).  Such a box spans the rows
+// of its tile's corners with at least one row of margin on each side; flat and
+// parallel rows take their extremes over a tile at its corners (linear-
+// fractional with a denominator of one sign over the volume, checked here, or
+// affine), and cylindrical rows k1 + k2 nv / rho can exceed their corners'
+// range inside a tile by at most |k2 nv| h^2 / (2 rho_min^3), h half a tile
+// edge (rows_inside_detector).  That bound, over the whole volume and every
+// view, must stay below half a row.
+bool box_inner_ok(const gct_plan &p, const std::vector<BpRec64> &rec64)
+{
+    const gct_ct_geom &g = p.geom;
+    if (g.kind != GCT_CYLINDRICAL && g.kind != GCT_FLAT) return true;
+    if (!p.circular || p.img.dim != 3) return false;
+    const double xm = double(p.img.x.num_samples) - 1.0, ym = double(p.img.y.num_samples) - 1.0;
+    const double zm = double((p.img.z.num_samples + 31u) / 32u * 32u) - 1.0;
+    const double X[2] = {0.0, xm}, Y[2] = {0.0, ym}, Z[2] = {0.0, zm};
+    for (const BpRec64 &r : rec64) {
+        auto form = [](const double f[4], double x, double y, double z) { return f[0] + f[1] * x + f[2] * y + f[3] * z; };
+        if (g.kind == GCT_FLAT) {
+            // the denominator of the flat rows keeps one sign over the volume
+            double wmin = 1e300, wmax = -1e300;
+            for (double x : X)
+                for (double y : Y)
+                    for (double z : Z) {
+                        const double nw = form(r.w, x, y, z);
+                        wmin = std::min(wmin, nw);
+                        wmax = std::max(wmax, nw);
+                    }
+            if (!(wmin > 0.0 || wmax < 0.0)) return false;
+            continue;
+        }
+        double pu[4], pw[4], nvmax = 0.0;
+        for (int c = 0; c < 4; ++c) {
+            pu[c] = form(r.u, X[c & 1], Y[c >> 1], 0.0);
+            pw[c] = form(r.w, X[c & 1], Y[c >> 1], 0.0);
+        }
+        for (double x : X)
+            for (double y : Y)
+                for (double z : Z) nvmax = std::max(nvmax, std::fabs(form(r.v, x, y, z)));
+        // rho's minimum over the footprint (a parallelogram in (nu, nw)): zero when it holds the origin
+        const int edge[4][2] = {{0, 1}, {1, 3}, {3, 2}, {2, 0}};
+        double rho_min = 1e300, cross_sign = 0.0;
+        bool inside = true;
+        for (const auto &e : edge) {
+            const double ax = pu[e[0]], ay = pw[e[0]], bx = pu[e[1]], by = pw[e[1]];
+            const double ex = bx - ax, ey = by - ay, len2 = ex * ex + ey * ey;
+            const double t = len2 > 0.0 ? std::clamp(-(ax * ex + ay * ey) / len2, 0.0, 1.0) : 0.0;
+            rho_min = std::min(rho_min, std::hypot(ax + t * ex, ay + t * ey));
+            const double cr = ax * ey - ay * ex;
+            if (cross_sign == 0.0) cross_sign = cr;
+            else if (cr * cross_sign <= 0.0) inside = false;
+        }
+        if (inside || !(rho_min > 0.0)) return false;
+        const double step = std::max(std::hypot(r.u[1], r.w[1]), std::hypot(r.u[2], r.w[2]));
+        const double h = 7.5 * step;
+        const double dev = std::fabs(r.k[2]) * nvmax * h * h / (rho_min * rho_min * rho_min);
+        if (!(dev < 0.5)) return false;
+    }
+    return true;
+}
+
 // Whether the arc kinds' FP32 kernels may take the fan angle without octant
 // reduction (fan_atan_narrow): in every view every voxel lies in front of the
 // source (nw > 0: nw is affine over the image, so its minimum is at a corner of
@@ -506,6 +568,7 @@ std::unique_ptr<gct_plan> make_plan(const gct_ct_geom &geom, const gct_image_geo
     build_records(*p, rec, rec64, frec);
     p->rows_inside = rows_inside_detector(*p, rec64);
     p->narrow_fan = narrow_fan_ok(*p, rec64);
+    p->box_inner = box_inner_ok(*p, rec64);
     const bool shape_free = !o.z_unroll && !o.view_block && !o.lds_rows && !o.lds_pairs;
     const int auto_shape = (shape_free && variant == GCT_BP_LDS && p->circular && !is_1d(geom.kind))
                                ? default_lds_shape(*p, rec64)
@@ -693,6 +756,7 @@ BpLaunch bp_args(gct_plan &p, float *vol, uint32_t first, uint32_t n, bool accum
     a.lds_pairs = p.lds_pairs;
     a.rows_inside = p.rows_inside;
     a.narrow_fan = p.narrow_fan;
+    a.box_inner = p.box_inner;
     a.round_robin = p.round_robin;
     return a;
 }
@@ -850,6 +914,7 @@ extern "C" gct_status gct_plan_get_info(const gct_plan *p, gct_plan_info *info)
         info->rows_inside = p->rows_inside;
         info->round_robin = lds && p->round_robin;
         info->narrow_fan = p->narrow_fan;
+        info->box_inner = p->box_inner;
     });
 }
 

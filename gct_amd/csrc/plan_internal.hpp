@@ -84,6 +84,7 @@ struct gct_plan {
     bool circular = true;
     bool rows_inside = false;   // every voxel projects inside the detector rows (the LDS kernel drops its row clamp)
     bool narrow_fan = false;    // arc kinds: the kernels' fan angle needs no octant reduction (narrow_fan_ok)
+    bool box_inner = false;     // the LDS kernel may drop the row clamp for boxes the detector did not cut (box_inner_ok)
     bool setup_f64 = false;
     float inv_dgamma = 0.0f;
     float scale = 1.0f;

@@ -267,6 +267,9 @@ typedef struct gct_plan_info {
     int32_t narrow_fan;              /* arc and cylindrical detectors, FP32 set-up: 1 when every voxel lies
                                         in front of the source in every view and the detector spans at most
                                         +-0.6 rad: the kernels take the fan angle without octant reduction */
+    int32_t box_inner;               /* 1 when the LDS backprojection may leave the row clamp out for a view
+                                        whose staged box the detector's edges did not cut (every voxel of a
+                                        tile then projects inside the box's rows) */
 } gct_plan_info;
 
 /* Builds per-view tables (FP64 on the host, stored FP32), the redundancy

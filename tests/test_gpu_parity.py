@@ -115,6 +115,34 @@ def test_fan_angle_forms(torch_dev, kind, form):
     assert O.are_equal(got, want, EPS).all()
 
 
+def test_box_inner_bound(torch_dev):
+    """The LDS kernel drops the row clamp for a box the detector's edges did
+    not cut only when the plan bounds how far a tile's interior rows exceed
+    its corners' (gct_plan_info.box_inner).  The cylindrical slab qualifies; a
+    cylindrical source 36 mm from the axis of a 32 mm wide volume does not
+    (tiles seen from a few centimetres bulge by more than half a row), and
+    that plan keeps the clamp and still matches the oracle."""
+    slab = W.cylindrical_slab(0)
+    plan = g.Plan(slab.scan, slab.img)
+    assert plan.info().box_inner == 1 and plan.info().rows_inside == 0
+    plan.close()
+    n, nu, nv, views = 64, 160, 40, 90
+    scan = g.circular_scan(g.CYLINDRICAL, nu, 0.012, views, 2.0 * math.pi / views, nv=nv, dv=1.0,
+                           det_row_center=(nu - 1) / 2.0, det_col_center=(nv - 1) / 2.0,
+                           src_det_dist=72.0, src_iso_dist=36.0)
+    img = g.Image(n, n, 0.5, 0.5, 32, 0.5, mid=(0.3, -0.2, 0.1))
+    ph = g.phantom_random(93, 8, 3, 14.0, (0.05, 0.35), (-0.02, 0.1))
+    wl = W.Workload("cyl_close", scan, img, ph, 0.0, 93)
+    proj = wl.projections()
+    got, plan = run_plan(torch_dev, wl, proj, bp_kernel=g.BP_LDS)
+    assert plan.info().box_inner == 0
+    want = O.fdk(wl.scan, wl.img, proj)
+    _report("cylindrical close source", got, want)
+    assert O.are_equal(got, want, EPS).all()
+    direct, _ = run_plan(torch_dev, wl, proj, bp_kernel=g.BP_DIRECT)
+    assert np.array_equal(direct, got)
+
+
 def test_parallel_1d_256_parity(torch_dev):
     """The 2-D parallel-beam workload (256^2, 360 views, 367 bins) in full."""
     wl = W.parallel_1d_256()
