@@ -484,6 +484,47 @@ __device__ inline void bp_tile3d(const BpLaunch &a, uint32_t bx, uint32_t by, ui
         if (static_cast<uint32_t>(k) < zn) vp[k * slice] = a.finish ? acc[k] * sc : acc[k];
 }
 
+// Column (padded, unclamped) and weight of pixel (fx, fy) in the view of
+// record r, 1-D detectors.  NARROW: the arc's fan angle without octant
+// reduction (BpLaunch::narrow_fan).
+template <int KIND, bool NARROW>
+__device__ inline void col2d(const BpLaunch &a, const BpRec &r, float fx, float fy, float &col, float &w)
+{
+    if constexpr (KIND == GCT_PARALLEL_1D) {
+        col = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
+        w = 1.0f;
+    } else if constexpr (KIND == GCT_FAN_LINE_1D) {
+        const float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
+        const float nu = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
+        const float rd = __builtin_amdgcn_rcpf(nw);   // v_rcp_f32 (1 ulp), as for flat 3-D
+        col = fmaf(nu, rd, r.k[0]);
+        w = rd * rd;
+    } else {   // GCT_FAN_ARC_1D
+        const float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
+        const float nu = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
+        col = fmaf(NARROW ? fan_atan_narrow(nu, nw) : fan_atan2(nu, nw), a.inv_dgamma, r.k[0]);
+        w = __builtin_amdgcn_rcpf(fmaf(nu, nu, nw * nw));   // forms scaled by 1/sqrt(SID)
+    }
+}
+
+// Calls f(std::bool_constant<NARROW>) with the launch's fan-angle form, so
+// that a view loop carries no per-view branch on it.
+template <int KIND, class F>
+__device__ inline void with_fan_form(const BpLaunch &a, F &&f)
+{
+    if constexpr (KIND == GCT_FAN_ARC_1D) {
+        if (a.narrow_fan) f(std::true_type{});
+        else f(std::false_type{});
+    } else {
+        f(std::false_type{});
+    }
+}
+
+// Views per group in the 2-D view loop: a group's taps are all requested
+// before the first is accumulated, so their latencies overlap; the sum still
+// runs view by view, so the result does not depend on the grouping.
+constexpr uint32_t kViews2d = 8;
+
 // One tile of a 2-D image (1-D detector): pixel (ix, iy) per thread.
 template <int KIND, int S>
 __device__ inline void bp_tile2d(const BpLaunch &a, uint32_t ix, uint32_t iy)
@@ -497,28 +538,27 @@ __device__ inline void bp_tile2d(const BpLaunch &a, uint32_t ix, uint32_t iy)
     const size_t view_elems = static_cast<size_t>(a.rows_pv) * a.pitch;
     const T *base = static_cast<const T *>(a.filt);
     const uint32_t vend = a.first_view + a.num_views;
-    for (uint32_t v = a.first_view; v < vend; ++v) {
-        const BpRec &r = a.rec[v];
-        const T *p = base + static_cast<size_t>(v) * view_elems;
-        float col, w;
-        if constexpr (KIND == GCT_PARALLEL_1D) {
-            col = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
-            w = 1.0f;
-        } else if constexpr (KIND == GCT_FAN_LINE_1D) {
-            const float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
-            const float nu = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
-            const float rd = __builtin_amdgcn_rcpf(nw);   // v_rcp_f32 (1 ulp), as for flat 3-D
-            col = fmaf(nu, rd, r.k[0]);
-            w = rd * rd;
-        } else {   // GCT_FAN_ARC_1D
-            const float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
-            const float nu = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
-            col = fmaf(fan_angle(a.narrow_fan, nu, nw), a.inv_dgamma, r.k[0]);
-            w = __builtin_amdgcn_rcpf(fmaf(nu, nu, nw * nw));   // forms scaled by 1/sqrt(SID)
+    with_fan_form<KIND>(a, [&](auto narrow) {
+        constexpr bool kNarrow = decltype(narrow)::value;
+        auto sample = [&](uint32_t v, float &w) {
+            float col;
+            col2d<KIND, kNarrow>(a, a.rec[v], fx, fy, col, w);
+            return linear<S>(base + static_cast<size_t>(v) * view_elems, a.pitch, col_sample(col, nup));
+        };
+        uint32_t v = a.first_view;
+        for (; v + kViews2d <= vend; v += kViews2d) {
+            float val[kViews2d], w[kViews2d];
+#pragma unroll
+            for (uint32_t j = 0; j < kViews2d; ++j) val[j] = sample(v + j, w[j]);
+#pragma unroll
+            for (uint32_t j = 0; j < kViews2d; ++j) acc = fmaf(val[j], w[j], acc);
         }
-        const float val = linear<S>(p, a.pitch, col_sample(col, nup));
-        acc = fmaf(val, w, acc);
-    }
+        for (; v < vend; ++v) {
+            float w;
+            const float val = sample(v, w);
+            acc = fmaf(val, w, acc);
+        }
+    });
     *vp = a.finish ? acc * a.scale : acc;
 }
 
@@ -1201,29 +1241,16 @@ __global__ __launch_bounds__(256) void bp2d_lds_kernel(BpLaunch a, uint32_t vb)
         }
         __syncthreads();
         if (active) {
-            for (uint32_t j = 0; j < nvb; ++j) {
-                const BpRec &r = a.rec[vb0 + j];
-                float col, w;
-                if constexpr (KIND == GCT_PARALLEL_1D) {
-                    col = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
-                    w = 1.0f;
-                } else if constexpr (KIND == GCT_FAN_LINE_1D) {
-                    const float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
-                    const float nu = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
-                    const float rd = __builtin_amdgcn_rcpf(nw);   // v_rcp_f32 (1 ulp), as for flat 3-D
-                    col = fmaf(nu, rd, r.k[0]);
-                    w = rd * rd;
-                } else {
-                    const float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
-                    const float nu = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
-                    col = fmaf(fan_angle(a.narrow_fan, nu, nw), a.inv_dgamma, r.k[0]);
-                    w = __builtin_amdgcn_rcpf(fmaf(nu, nu, nw * nw));   // forms scaled by 1/sqrt(SID)
+            with_fan_form<KIND>(a, [&](auto narrow) {
+                for (uint32_t j = 0; j < nvb; ++j) {
+                    float col, w;
+                    col2d<KIND, decltype(narrow)::value>(a, a.rec[vb0 + j], fx, fy, col, w);
+                    const Sample cs = col_sample(col, nup);
+                    const float *rr = rows + j * a.pitch + cs.colbase;
+                    const float t0 = rr[0], t1 = rr[1];
+                    acc = fmaf(fmaf(cs.alpha, t1 - t0, t0), w, acc);
                 }
-                const Sample cs = col_sample(col, nup);
-                const float *rr = rows + j * a.pitch + cs.colbase;
-                const float t0 = rr[0], t1 = rr[1];
-                acc = fmaf(fmaf(cs.alpha, t1 - t0, t0), w, acc);
-            }
+            });
         }
         __syncthreads();
     }
