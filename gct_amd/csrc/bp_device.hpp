@@ -24,17 +24,17 @@ namespace {
 template <int S> struct Stor;
 template <> struct Stor<GCT_STORE_F32> {
     using T = float;
-    __device__ static float ld(const T *p, uint32_t i) { return p[i]; }
+    template <class P> __device__ static float ld(P p, uint32_t i) { return p[i]; }
     __device__ static T cv(float x) { return x; }
 };
 template <> struct Stor<GCT_STORE_F16> {
     using T = _Float16;
-    __device__ static float ld(const T *p, uint32_t i) { return static_cast<float>(p[i]); }
+    template <class P> __device__ static float ld(P p, uint32_t i) { return static_cast<float>(p[i]); }
     __device__ static T cv(float x) { return static_cast<_Float16>(x); }
 };
 template <> struct Stor<GCT_STORE_BF16> {
     using T = uint16_t;
-    __device__ static float ld(const T *p, uint32_t i) { return __uint_as_float(uint32_t(p[i]) << 16); }
+    template <class P> __device__ static float ld(P p, uint32_t i) { return __uint_as_float(uint32_t(p[i]) << 16); }
     __device__ static T cv(float x)
     {
         uint32_t u = __float_as_uint(x);
@@ -42,6 +42,37 @@ template <> struct Stor<GCT_STORE_BF16> {
         return static_cast<uint16_t>(u >> 16);
     }
 };
+
+// Global-memory forms of a pointer.  Pointers that a kernel reads from a work
+// table in memory (the persistent batch) are generic to the compiler, which
+// then takes flat loads for what they address, and a wave-uniform load that a
+// store earlier in the kernel might have clobbered stays a vector load.  Loads
+// through GlobalPtr are global loads; through ConstPtr (the constant address
+// space: memory no thread writes during the kernel, such as the view
+// records) wave-uniform loads are scalar loads.
+// (The host pass of a .hip file only type-checks these device functions: there
+// the types are plain pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using GlobalPtr = __attribute__((address_space(1))) T *;
+template <class T>
+using ConstPtr = __attribute__((address_space(4))) const T *;
+#else
+template <class T>
+using GlobalPtr = T *;
+template <class T>
+using ConstPtr = const T *;
+#endif
+template <class T>
+__device__ inline GlobalPtr<T> to_global(T *p)
+{
+    return (GlobalPtr<T>)p;
+}
+template <class T>
+__device__ inline ConstPtr<T> to_const(const T *p)
+{
+    return (ConstPtr<T>)p;
+}
 
 // The horizontal difference t1 - t0 of two neighbouring filtered values, as
 // every backprojection path takes it: in FP32, and for FP16 storage rounded to
@@ -220,7 +251,7 @@ __device__ inline ColSetup setup_column(const BpLaunch &a, uint32_t v, float fx,
         // The column is an affine form with a large constant (the detector
         // centre); evaluating it in FP64 once per column keeps it as accurate
         // as the reference's FP32 chain t = x cos + y sin, col = t / du + cu.
-        const BpRec64 &r = a.rec64[v];
+        const BpRec64 r = to_const(a.rec64)[v];
         s.col = static_cast<float>(fma(r.u[2], static_cast<double>(fy), fma(r.u[1], static_cast<double>(fx), r.u[0])));
         s.w = 1.0f;
         s.row0d = Z ? fma(r.v[3], static_cast<double>(fz), r.v[0]) : r.v[0];
@@ -229,7 +260,7 @@ __device__ inline ColSetup setup_column(const BpLaunch &a, uint32_t v, float fx,
         s.drow = static_cast<float>(r.v[3]);
     } else if constexpr (KIND == GCT_FLAT) {
         if constexpr (F64) {
-            const BpRec64 &r = a.rec64[v];
+            const BpRec64 r = to_const(a.rec64)[v];
             const double X = fx, Y = fy, W = fz;
             double nw = fma(r.w[2], Y, fma(r.w[1], X, r.w[0]));
             double nu = fma(r.u[2], Y, fma(r.u[1], X, r.u[0]));
@@ -252,7 +283,7 @@ __device__ inline ColSetup setup_column(const BpLaunch &a, uint32_t v, float fx,
             s.row0 = static_cast<float>(s.row0d);
             s.drow = static_cast<float>(s.drowd);
         } else {
-            const BpRec &r = a.rec[v];
+            const BpRec r = to_const(a.rec)[v];
             float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
             float nu = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
             float nv = fmaf(r.v[2], fy, fmaf(r.v[1], fx, r.v[0]));
@@ -269,7 +300,7 @@ __device__ inline ColSetup setup_column(const BpLaunch &a, uint32_t v, float fx,
         }
     } else if constexpr (KIND == GCT_CYLINDRICAL) {
         if constexpr (F64) {
-            const BpRec64 &r = a.rec64[v];
+            const BpRec64 r = to_const(a.rec64)[v];
             const double X = fx, Y = fy, W = fz;
             double nw = fma(r.w[2], Y, fma(r.w[1], X, r.w[0]));
             double nu = fma(r.u[2], Y, fma(r.u[1], X, r.u[0]));
@@ -292,7 +323,7 @@ __device__ inline ColSetup setup_column(const BpLaunch &a, uint32_t v, float fx,
             s.dnv = static_cast<float>(r.v[3]);
             s.w = 1.0f;   // the forms are scaled so that the weight is 1 / |r|^2
         } else {
-            const BpRec &r = a.rec[v];
+            const BpRec r = to_const(a.rec)[v];
             float nw = fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0]));
             float nu = fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0]));
             float nv = fmaf(r.v[2], fy, fmaf(r.v[1], fx, r.v[0]));
@@ -376,18 +407,18 @@ __device__ inline NcBase nc_base(const BpLaunch &a, uint32_t v, float fx, float 
 {
     NcBase b;
     if constexpr (F64) {
-        const BpRec64 &r = a.rec64[v];
+        const BpRec64 r = to_const(a.rec64)[v];
         const double X = fx, Y = fy, Z = zb;
         b.nw = static_cast<float>(fma(r.w[3], Z, fma(r.w[2], Y, fma(r.w[1], X, r.w[0]))));
         b.nu = static_cast<float>(fma(r.u[3], Z, fma(r.u[2], Y, fma(r.u[1], X, r.u[0]))));
         b.nv = static_cast<float>(fma(r.v[3], Z, fma(r.v[2], Y, fma(r.v[1], X, r.v[0]))));
     } else {
-        const BpRec &r = a.rec[v];
+        const BpRec r = to_const(a.rec)[v];
         b.nw = fmaf(r.w[3], zb, fmaf(r.w[2], fy, fmaf(r.w[1], fx, r.w[0])));
         b.nu = fmaf(r.u[3], zb, fmaf(r.u[2], fy, fmaf(r.u[1], fx, r.u[0])));
         b.nv = fmaf(r.v[3], zb, fmaf(r.v[2], fy, fmaf(r.v[1], fx, r.v[0])));
     }
-    const BpRec &r = a.rec[v];
+    const BpRec r = to_const(a.rec)[v];
     b.dw = r.w[3];
     b.du = r.u[3];
     b.dv = r.v[3];
@@ -531,19 +562,26 @@ __device__ inline void bp_tile2d(const BpLaunch &a, uint32_t ix, uint32_t iy)
 {
     using T = typename Stor<S>::T;
     if (ix >= a.nx || iy >= a.ny) return;
-    float *vp = a.vol + static_cast<size_t>(iy) * a.nx + ix;
+    GlobalPtr<float> vp = to_global(a.vol) + static_cast<size_t>(iy) * a.nx + ix;
     float acc = a.accumulate ? *vp : 0.0f;
     const float fx = static_cast<float>(ix), fy = static_cast<float>(iy);
     const float nup = static_cast<float>(a.nu + 1u);
     const size_t view_elems = static_cast<size_t>(a.rows_pv) * a.pitch;
-    const T *base = static_cast<const T *>(a.filt);
+    const GlobalPtr<const T> base = to_global(static_cast<const T *>(a.filt)) + a.pitch;   // row 0 of view 0
+    const ConstPtr<BpRec> rec = to_const(a.rec);
     const uint32_t vend = a.first_view + a.num_views;
     with_fan_form<KIND>(a, [&](auto narrow) {
         constexpr bool kNarrow = decltype(narrow)::value;
+        // linear<S> on the view's row: taps at a 32-bit byte offset from the row
         auto sample = [&](uint32_t v, float &w) {
+            const BpRec r = rec[v];
             float col;
-            col2d<KIND, kNarrow>(a, a.rec[v], fx, fy, col, w);
-            return linear<S>(base + static_cast<size_t>(v) * view_elems, a.pitch, col_sample(col, nup));
+            col2d<KIND, kNarrow>(a, r, fx, fy, col, w);
+            const Sample cs = col_sample(col, nup);
+            const auto q = (GlobalPtr<const T>)((GlobalPtr<const char>)(base + static_cast<size_t>(v) * view_elems) +
+                                                cs.colbase * static_cast<uint32_t>(sizeof(T)));
+            const float t0 = Stor<S>::ld(q, 0), t1 = Stor<S>::ld(q, 1);
+            return fmaf(cs.alpha, t1 - t0, t0);
         };
         uint32_t v = a.first_view;
         for (; v + kViews2d <= vend; v += kViews2d) {
