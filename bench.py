@@ -433,17 +433,9 @@ def run_precision(a, g, W, torch, np):
         plan.execute(p, v, stream=stream)
         torch.cuda.synchronize()
         plan.autotune(v)          # each storage form picks its own backprojection variant
-        ts = []
-        for _ in range(3):
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            plan.execute(p, v, stream=stream)
-            e1.record(stream)
-            torch.cuda.synchronize()
-            ts.append(e0.elapsed_time(e1))
-        ms = statistics.median(ts)
+        ms = statistics.median(timed_events(torch, stream, lambda: plan.execute(p, v, stream=stream), 5))
         info = plan.info()
-        bp_ms = statistics.median(timed_events(torch, stream, lambda: plan.backproject(v, stream=stream), 3))
+        bp_ms = statistics.median(timed_events(torch, stream, lambda: plan.backproject(v, stream=stream), 5))
         plan.execute(p, v, stream=stream)
         sbytes = 4 if st == g.STORE_F32 else 2
         alg_bytes = wl.scan.num_proj * wl.scan.nv * wl.scan.nu * sbytes + wl.img.nvox * 4
