@@ -220,18 +220,47 @@ constexpr uint32_t last_radix()
         return last_radix<L, P * next_radix<L / P>()>();
 }
 
+// Whether the group buffers of an L-point filter with G row pairs per
+// workgroup are padded between the first two passes, both radix 16.  Lane i
+// of the first pass writes points 16 i .. 16 i + 15, so the lanes of one store
+// are 16 points (128 bytes) apart and meet in the same LDS banks (half of the
+// 768-point filter's LDS cycles were bank conflicts).  Padded, point m of that
+// buffer sits in slot m + m / 16: the lanes' stores are 17 slots apart.  Only
+// where the larger buffers leave as many workgroups per CU (LDS-limited) as
+// the unpadded ones.
+template <uint32_t L, uint32_t G>
+constexpr bool fft_pad()
+{
+    constexpr uint32_t lds = 160u * 1024u;
+    constexpr uint32_t plain = (G * L + L) * 8u, padded = (G * (L + L / 16u) + L) * 8u;
+    if constexpr (L % 256u != 0u || next_radix<L>() != 16u || next_radix<L / 16u>() != 16u) return false;
+    else return lds / padded == lds / plain;
+}
+
 // The passes of an L-point FFT from product P on, in the group's LDS buffer
 // `buf`.  The first pass reads through `ld` when LD_CUSTOM (else from
 // `buf`); the last always writes through `st`, which writes `buf` when
 // ST_LDS.  Passes that write `buf` end with a barrier for its next reader.
-template <uint32_t L, uint32_t P, uint32_t NT, bool LD_CUSTOM, bool ST_LDS, class Ld, class St>
+// PAD: the first pass writes `buf` padded (fft_pad) and the second reads it so.
+template <uint32_t L, uint32_t P, uint32_t NT, bool LD_CUSTOM, bool ST_LDS, bool PAD, class Ld, class St>
 __device__ inline void fft_io(float2 *buf, const float2 *tw, uint32_t tid, Ld ld, St st)
 {
     if constexpr (P < L) {
         constexpr uint32_t R = next_radix<L / P>();
         constexpr bool first = P == 1u, last = P * R == L;
-        auto lds_ld = [buf](uint32_t i, uint32_t j) { return buf[i + j * (L / R)]; };
-        auto lds_st = [buf](uint32_t idx, uint32_t, float2 v) { buf[idx] = v; };
+        // padded slots: written by the first pass (point 16 i + j of lane i in
+        // slot 17 i + j), read by the second (point i + j L / 16 in slot
+        // i + i / 16 + j (L / 16 + L / 256)), both as a per-lane base plus
+        // constant offsets
+        constexpr bool pad_out = PAD && first, pad_in = PAD && P == 16u;
+        auto lds_ld = [buf](uint32_t i, uint32_t j) {
+            if constexpr (pad_in) return buf[i + (i >> 4) + j * (L / R + L / R / 16u)];
+            else return buf[i + j * (L / R)];
+        };
+        auto lds_st = [buf](uint32_t idx, uint32_t, float2 v) {
+            if constexpr (pad_out) buf[idx + (idx >> 4)] = v;
+            else buf[idx] = v;
+        };
         constexpr bool in_lds = !(first && LD_CUSTOM), out_lds = !last || ST_LDS;
         constexpr bool sync_mid = in_lds && out_lds, sync_end = out_lds;
         if constexpr (first && LD_CUSTOM) {
@@ -245,7 +274,7 @@ __device__ inline void fft_io(float2 *buf, const float2 *tw, uint32_t tid, Ld ld
             else
                 stockham_io<L, R, P, NT, sync_mid, sync_end>(tw, tid, lds_ld, lds_st);
         }
-        fft_io<L, P * R, NT, LD_CUSTOM, ST_LDS>(buf, tw, tid, ld, st);
+        fft_io<L, P * R, NT, LD_CUSTOM, ST_LDS, PAD>(buf, tw, tid, ld, st);
     }
 }
 
@@ -293,8 +322,9 @@ template <int KIND, int S, uint32_t L>
 __global__ __launch_bounds__(kFiltBlock) void filter_kernel(FilterLaunch a)
 {
     constexpr uint32_t NT = filt_nt<L>(), G = kFiltBlock / NT;
+    constexpr bool kPad = fft_pad<L, G>();
     __shared__ float2 tw[L];
-    __shared__ float2 bufs[G][L];
+    __shared__ float2 bufs[G][kPad ? L + L / 16u : L];
     using St = Stor<S>;
     for (uint32_t i = threadIdx.x; i < L; i += kFiltBlock) tw[i] = a.tw[i];
     const uint32_t grp = threadIdx.x / NT, tid = threadIdx.x % NT;
@@ -336,7 +366,7 @@ __global__ __launch_bounds__(kFiltBlock) void filter_kernel(FilterLaunch a)
     };
     // (the first pass uses no twiddles: the barrier that ends it also
     // publishes the table)
-    fft_io<L, 1, NT, true, true>(buf, tw, tid, ld_rows, st_spectrum);
+    fft_io<L, 1, NT, true, true, kPad>(buf, tw, tid, ld_rows, st_spectrum);
 
     // Inverse FFT (conj . FFT . conj): the last pass writes the filtered rows
     // straight into the padded layout; only points below nu are kept.
@@ -353,7 +383,7 @@ __global__ __launch_bounds__(kFiltBlock) void filter_kernel(FilterLaunch a)
         }
     };
     auto ld_none = [](uint32_t, uint32_t) { return make_float2(0.0f, 0.0f); };
-    fft_io<L, 1, NT, false, false>(buf, tw, tid, ld_none, st_rows);
+    fft_io<L, 1, NT, false, false, kPad>(buf, tw, tid, ld_none, st_rows);
 }
 
 __global__ void zero_counters_kernel(uint32_t *c, uint32_t n)
