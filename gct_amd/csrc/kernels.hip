@@ -294,19 +294,25 @@ __device__ inline float cos_weight(const FiltRec &f, float du, float dv, uint32_
 }
 
 // Threads per row pair: L below 64, one wavefront up to L = 1024, then L / 16
-// (one radix-16 butterfly per thread and pass); for L = 3 * 2^k, the power of
-// two from 64 that covers the L / 16 butterflies of a radix-16 pass.  A
-// 256-thread workgroup holds 256 / NT row pairs, which share the LDS twiddle
-// table.
+// (one radix-16 butterfly per thread and pass); for L = 3 * 2^k, L / 16 from
+// 768 to 1536 points (48 or 96 threads: every thread busy in the radix-16
+// passes), else the power of two from 64 that covers the L / 16 butterflies
+// of a radix-16 pass.  A workgroup holds the 256 / NT row pairs that fit 256
+// threads (filt_block), which share the LDS twiddle table.
 template <uint32_t L>
 constexpr uint32_t filt_nt()
 {
     if constexpr (L % 3u == 0u)
-        return L <= 1024u ? 64u : (L <= 2048u ? 128u : 256u);
+        return (L == 768u || L == 1536u) ? L / 16u : (L <= 1024u ? 64u : (L <= 2048u ? 128u : 256u));
     else
         return L < 64u ? L : (L <= 1024u ? 64u : L / 16u);
 }
 constexpr uint32_t kFiltBlock = 256;
+template <uint32_t L>
+constexpr uint32_t filt_block()
+{
+    return kFiltBlock / filt_nt<L>() * filt_nt<L>();
+}
 
 // One group of NT threads filters two detector rows as one complex signal
 // z = a + i b: IFFT(FFT(z) H) = (a * h) + i (b * h) because the kernel h is
@@ -323,7 +329,7 @@ __global__ __launch_bounds__(kFiltBlock) void filter_kernel(FilterLaunch a)
     __shared__ float2 tw[L];
     __shared__ float2 bufs[G][kPad ? L + L / 16u : L];
     using St = Stor<S>;
-    for (uint32_t i = threadIdx.x; i < L; i += kFiltBlock) tw[i] = a.tw[i];
+    for (uint32_t i = threadIdx.x; i < L; i += filt_block<L>()) tw[i] = a.tw[i];
     const uint32_t grp = threadIdx.x / NT, tid = threadIdx.x % NT;
     float2 *buf = bufs[grp];
     const uint32_t nv = a.nv, nu = a.nu;
@@ -1043,10 +1049,10 @@ void launch_filter_L(const FilterLaunch &a, hipStream_t s)
     const uint32_t nrows = a.num_views * (is_1d(KIND) ? 1u : a.nv);
     const uint32_t pairs = (nrows + 1u) / 2u;
     if (pairs == 0) return;
-    // workgroups of 256 threads, 256 / filt_nt<L>() row pairs each
+    // workgroups of 256 / filt_nt<L>() row pairs, filt_nt<L>() threads each
     auto go = [&](auto kern, uint32_t nt) {
         const uint32_t g = kFiltBlock / nt;
-        hipLaunchKernelGGL(kern, dim3((pairs + g - 1u) / g), dim3(kFiltBlock), 0, s, a);
+        hipLaunchKernelGGL(kern, dim3((pairs + g - 1u) / g), dim3(g * nt), 0, s, a);
     };
     switch (a.L) {
     case 48: go(filter_kernel<KIND, S, 48>, filt_nt<48>()); break;
