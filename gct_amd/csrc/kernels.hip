@@ -293,19 +293,20 @@ __device__ inline float cos_weight(const FiltRec &f, float du, float dv, uint32_
     return 1.0f;
 }
 
-// Threads per row pair: L below 64, one wavefront up to L = 1024, then L / 16
-// (one radix-16 butterfly per thread and pass); for L = 3 * 2^k, L / 16 from
-// 768 to 1536 points (48 or 96 threads: every thread busy in the radix-16
-// passes), else the power of two from 64 that covers the L / 16 butterflies
-// of a radix-16 pass.  A workgroup holds the 256 / NT row pairs that fit 256
-// threads (filt_block), which share the LDS twiddle table.
+// Threads per row pair: from 256 to 2048 points, L / 16 -- one radix-16
+// butterfly per thread and pass, so every thread is busy in those passes (64
+// threads left three quarters of them idle at 256 points: 0.034 against
+// 0.020 ms, a quarter at 768: 0.308 against 0.279 ms); beyond, 256 (192 at
+// 3072 points, one workgroup of 3 waves, was slower: 0.233 against 0.212 ms);
+// below, L under 64 and 64 from there.  A workgroup holds the 256 / NT row
+// pairs that fit 256 threads (filt_block), which share the LDS twiddle table.
 template <uint32_t L>
 constexpr uint32_t filt_nt()
 {
-    if constexpr (L % 3u == 0u)
-        return (L == 768u || L == 1536u) ? L / 16u : (L <= 1024u ? 64u : (L <= 2048u ? 128u : 256u));
+    if constexpr (L >= 256u)
+        return L / 16u <= 128u ? L / 16u : 256u;
     else
-        return L < 64u ? L : (L <= 1024u ? 64u : L / 16u);
+        return L < 64u ? L : 64u;
 }
 constexpr uint32_t kFiltBlock = 256;
 template <uint32_t L>
