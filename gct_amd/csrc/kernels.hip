@@ -894,8 +894,8 @@ void joseph_lds_kernel(FpLaunch a, uint32_t tiles_c, uint32_t tiles_r)
             // the box: elements past its columns, rows or planes land in the brick and are never
             // read (the traced taps stay within columns < W, rows < H and planes < np).  Skipping
             // the rows past the box (by scalar branches, or by an offset the range check drops)
-            // fetched a third of the bytes but took longer: the per-row selects cost more issue
-            // than the extra loads cost bandwidth.  The row's element index is wave-uniform
+            // fetched fewer bytes but took longer (measured with 16-row bricks): the per-row
+            // selects cost more issue than the extra loads cost bandwidth.  The row's element index is wave-uniform
             // (scalar); the loads are raw buffer loads with the whole byte offset in the lane's
             // (range-checked) operand, so an element past the copies reads 0 instead of faulting.
             // In two halves: 12 loads in flight per lane, then their stores, each at an immediate
